@@ -1,0 +1,201 @@
+/*
+ * qmcbeaver_amd.h -- C ABI of the MI355X walker-evaluation library.
+ *
+ * One context holds one molecule (basis, MO coefficients, Jastrow) plus the
+ * device-resident state of up to `max_walkers` walkers: positions, the FP64
+ * inverse Slater matrices, log-determinants, derivative matrices and Jastrow
+ * sums.  All entry points take plain pointers and sizes.
+ *
+ * Reference interface replaced (QMcBeaver, paths relative to QMcBeaver/):
+ *   class QMCFunctions            include/QMCFunctions.h:26-105
+ *   QMCFunctionsFactory           src/QMCFunctionsFactory.cpp:15-37
+ *   QMCSCFJastrow::initialize     src/QMCSCFJastrow.cpp:61-89
+ *   QMCSCFJastrow::evaluate       src/QMCSCFJastrow.cpp:91-192
+ *   QMCWalkerData outputs         include/QMCWalkerData.h:64-226
+ */
+#ifndef QMCBEAVER_AMD_H
+#define QMCBEAVER_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QMCB_ABI_VERSION 1
+#define QMCB_PADE_MAX 8
+
+typedef enum qmcb_status {
+  QMCB_OK = 0,
+  QMCB_INVALID_ARG = 1,
+  QMCB_OOM = 2,
+  QMCB_DEVICE_ERROR = 3,
+  QMCB_BAD_STATE = 4
+} qmcb_status;
+
+/* Arithmetic of the basis values and the orbital contraction.  The inverse
+ * Slater matrices, log-determinants, traces, Jastrow and energies are FP64
+ * in every mode. */
+typedef enum qmcb_precision {
+  QMCB_FP64 = 0,        /* everything FP64 (reference arithmetic)           */
+  QMCB_MIXED_FP32 = 1,  /* basis + orbital GEMM FP32 (MFMA, FP32 accumulate) */
+  QMCB_MIXED_BF16 = 2   /* basis FP32, GEMM operands BF16, FP32 accumulate  */
+} qmcb_precision;
+
+/* Drift (quantum force) modification, QMCSCFJastrow.cpp:329-472. */
+typedef enum qmcb_drift_mod {
+  QMCB_DRIFT_NONE = 0,
+  QMCB_DRIFT_UMRIGAR93_UNEQUAL = 1,
+  QMCB_DRIFT_UMRIGAR93_EQUAL = 2
+} qmcb_drift_mod;
+
+/* Pade correlation function u(r) = (num[0] r + num[1] r^2 + ...) /
+ * (1 + den[0] r + den[1] r^2 + ...).  Covers the reference's "Pade" and
+ * "FixedCuspPade" types (PadeCorrelationFunction.cpp:15-83,
+ * FixedCuspPadeCorrelationFunction.cpp:15-117): both are this rational form
+ * once constants and parameters are placed. */
+typedef struct qmcb_pade {
+  int32_t n_num;
+  int32_t n_den;
+  double num[QMCB_PADE_MAX];
+  double den[QMCB_PADE_MAX];
+} qmcb_pade;
+
+/* Flat, self-contained system description (copied at create time; no host
+ * pointer is retained).  Replaces the parts of the global QMCInput the
+ * evaluate path reads: QMCMolecule, QMCBasisFunction(Coefficients),
+ * QMCWavefunction::getCoeff, QMCJastrowParameters. */
+typedef struct qmcb_system {
+  int32_t n_up, n_dn;       /* electrons per spin, n_dn <= n_up; alpha first */
+  int32_t n_nuc;            /* nuclei                                          */
+  int32_t n_basis;          /* basis functions B                               */
+  int32_t n_prim;           /* total primitives                                */
+  const double *nuc_xyz;    /* [n_nuc][3] bohr                                 */
+  const double *nuc_z;      /* [n_nuc] (effective) charges                     */
+  const int32_t *bf_atom;   /* [n_basis] centre of each basis function         */
+  const int32_t *bf_pow;    /* [n_basis][3] Cartesian powers a,b,c (<= 2)      */
+  const int32_t *bf_prim_off; /* [n_basis+1] primitive ranges                  */
+  const double *prim_alpha; /* [n_prim] exponents                              */
+  const double *prim_coef;  /* [n_prim] raw contraction coefficients           */
+  const double *mo_up;      /* [n_up][n_basis] occupied orbitals, alpha        */
+  const double *mo_dn;      /* [n_dn][n_basis] occupied orbitals, beta         */
+  qmcb_pade ee_updn, ee_upup, ee_dndn;
+  const qmcb_pade *en_up;   /* [n_nuc] e-n function seen by alpha electrons    */
+  const qmcb_pade *en_dn;   /* [n_nuc] e-n function seen by beta electrons     */
+} qmcb_system;
+
+typedef struct qmcb_options {
+  int32_t precision;        /* qmcb_precision                                  */
+  int32_t max_walkers;      /* resident walker slots                           */
+  int32_t drift_mod;        /* qmcb_drift_mod                                  */
+  int32_t refresh_interval; /* FP64 re-inversion every K accepted-or-not moves
+                               per walker in single-electron mode; 0 = never   */
+  double dt;                /* time step used by the drift modification        */
+  double umrigar_a;         /* umrigar93_equalelectrons_parameter              */
+  uint64_t seed;            /* device RNG seed (vmc entry points)              */
+  int32_t device;           /* HIP device ordinal                              */
+  int32_t reserved;
+} qmcb_options;
+
+/* Caller-owned outputs, one row per evaluated walker.  Any pointer may be
+ * NULL (not written).  N = n_up + n_dn. */
+typedef struct qmcb_results {
+  double *log_psi;          /* [num] ln|Psi| = ln|D_up D_dn| + U                */
+  int8_t *sign_psi;         /* [num] sign of Psi                               */
+  double *grad_lnpsi;       /* [num][N][3] gradPsiRatio = D_x + U_x            */
+  double *mod_drift;        /* [num][N][3] modifiedGradPsiRatio                */
+  double *lap_ratio;        /* [num] Laplacian(Psi)/Psi                        */
+  double *e_local;          /* [num] local energy                              */
+  double *kinetic;          /* [num] -1/2 Laplacian(Psi)/Psi                    */
+  double *potential;        /* [num] V_nn + V_en + V_ee                        */
+  double *v_ne;             /* [num]                                           */
+  double *v_ee;             /* [num]                                           */
+  double *det_ratio;        /* [num] new/old determinant of the moved spin
+                               (single-electron evaluate); 1 for all-electron  */
+  double *jastrow_u;        /* [num] U                                         */
+  int8_t *singular;         /* [num] numerical failure flag                    */
+} qmcb_results;
+
+typedef struct qmcb_ctx qmcb_ctx;
+
+/* Library/ABI version; no device work. */
+int32_t qmcb_abi_version(void);
+
+/* Build a context: copies `sys` to the device and allocates the resident
+ * walker state.  Replaces QMCFunctionsFactory::functionsFactory +
+ * QMCSCFJastrow::initialize (QMCSCFJastrow.cpp:61-89). */
+qmcb_status qmcb_create(const qmcb_system *sys, const qmcb_options *opt,
+                        qmcb_ctx **out);
+
+void qmcb_destroy(qmcb_ctx *ctx);
+
+/* Last error message of this context (or of the last failed create when
+ * ctx is NULL). */
+const char *qmcb_last_error(const qmcb_ctx *ctx);
+
+/* Evaluate a batch of walkers: QMCFunctions::evaluate(walkerData, xData, num)
+ * (QMCFunctions.h:67-70, QMCSCFJastrow.cpp:91-181).
+ *   walker_ids [num]   resident slots
+ *   R          [num][N][3] host positions, alpha electrons first
+ *   whichE     -1: all-electron evaluate; (re)initialises the resident state
+ *              of each slot from R.
+ *              k>=0: single-electron proposal; only row k of R is read, the
+ *              other electrons are the slot's resident positions.  The
+ *              proposal is staged until qmcb_accept.
+ * Numerical failure sets results->singular, never an error status. */
+qmcb_status qmcb_evaluate(qmcb_ctx *ctx, int32_t num, const int32_t *walker_ids,
+                          const double *R, int32_t whichE, qmcb_results *out);
+
+/* Same as qmcb_evaluate with device-resident R and device-resident result
+ * arrays (every non-NULL pointer in `out` is a device pointer).  Asynchronous
+ * on the context's stream. */
+qmcb_status qmcb_evaluate_device(qmcb_ctx *ctx, int32_t num,
+                                 const int32_t *walker_ids_dev,
+                                 const double *R_dev, int32_t whichE,
+                                 qmcb_results *out_dev);
+
+/* Commit (mask[i]=1) or discard (mask[i]=0) the staged single-electron
+ * proposal of walker_ids[i].  Replaces the reference's re-evaluation at the
+ * old position after a rejected move (QMCWalker.cpp:254-266). */
+qmcb_status qmcb_accept(qmcb_ctx *ctx, int32_t num, const int32_t *walker_ids,
+                        const uint8_t *mask);
+
+/* FP64 re-inversion of the resident Slater matrices of the given slots
+ * (no reference counterpart in single-electron mode, which never re-inverts;
+ * the all-electron path is QMCSlater.cpp:675-683). */
+qmcb_status qmcb_refresh(qmcb_ctx *ctx, int32_t num, const int32_t *walker_ids);
+
+/* Copy resident positions of slots [0, num) to host R[num][N][3]. */
+qmcb_status qmcb_get_positions(qmcb_ctx *ctx, int32_t num, double *R);
+
+/* ---- device-resident VMC (the caller loop of the path) ------------------
+ * QMCRun::propagateWalkers + QMCWalker::initializePropagation /
+ * processPropagation with importance sampling (QMCRun.cpp:28-87,
+ * QMCWalker.cpp:179-282, 343-398, 756-780, 1578-1685): one step moves
+ * electron (step mod N) of every walker in slots [0, num), evaluates the
+ * full contract at the proposal, and accepts or rejects on the device with a
+ * counter-based RNG keyed by (seed, slot, step). */
+typedef struct qmcb_vmc_stats {
+  double sum_e_local;       /* sum over walkers and steps of E_L (current cfg) */
+  double sum_e_local_sq;
+  double sum_accept;        /* accepted moves                                  */
+  double sum_prop_e_local;  /* sum of E_L at the proposals                     */
+  int64_t n_samples;
+} qmcb_vmc_stats;
+
+qmcb_status qmcb_vmc_steps(qmcb_ctx *ctx, int32_t num, int64_t first_step,
+                           int32_t n_steps, double tau, qmcb_vmc_stats *stats);
+
+qmcb_status qmcb_sync(qmcb_ctx *ctx);
+
+/* Per-kernel timing of the last qmcb_vmc_steps call, measured with HIP
+ * events on the context's stream.  names/ms arrays of length cap; returns
+ * the number of entries. */
+int32_t qmcb_kernel_times(qmcb_ctx *ctx, int32_t cap, const char **names,
+                          double *ms, int64_t *launches);
+void qmcb_enable_kernel_timing(qmcb_ctx *ctx, int32_t on);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QMCBEAVER_AMD_H */

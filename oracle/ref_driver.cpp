@@ -1,0 +1,221 @@
+// ref_driver.cpp -- this project's driver around the reference's own
+// evaluate path, compiled from the reference sources by oracle/ref.mk into
+// oracle/_ref/ (git-ignored).  Test infrastructure: it writes the golden
+// vectors under tests/golden/ and times the reference CPU path.
+//
+// It calls, unchanged, QMCInput::read (QMCInput.cpp:26-53),
+// QMCSCFJastrow::initialize/evaluate (QMCSCFJastrow.cpp:61-192) and
+// QMCWalkerData::initialize/updateDistances (QMCWalkerData.cpp:56-161,
+// 302-346).  Walker records are freshly initialised for every all-electron
+// evaluation (zero Jastrow caches).
+//
+//   qmcb_ref_driver eval <input.ckmf> <configs.bin> <out.bin>
+//   qmcb_ref_driver seq  <input.ckmf> <moves.bin>   <out.bin>
+//   qmcb_ref_driver vmc  <input.ckmf> <configs.bin> <n_moves> <tau> <seed>
+//
+// configs.bin: int32 nconf, int32 N, double R[nconf][N][3]
+// moves.bin:   int32 nwalk, int32 N, int32 nmoves, then per walker
+//              double R[N][3] followed by nmoves x double (e, x, y, z)
+// out.bin:     per evaluation one record of 11 + 6N doubles:
+//              detA detB U E_L KE PE V_ne V_ee singular modRatio lnPsi,
+//              gradPsiRatio[N][3], modifiedGradPsiRatio[N][3]
+#include <cstdio>
+#include <cstdlib>
+#include <cmath>
+#include <vector>
+#include <sys/time.h>
+
+#include "QMCInput.h"
+#include "QMCSCFJastrow.h"
+#include "QMCWalkerData.h"
+
+QMCInput globalInput;
+
+static void record(std::FILE *f, QMCWalkerData &wd, int N) {
+  int na = globalInput.WF.getNumberElectrons(true);
+  int nb = globalInput.WF.getNumberElectrons(false);
+  double detA = na > 0 ? wd.DcA(0) : 1.0;
+  double detB = nb > 0 ? wd.DcB(0) : 1.0;
+  std::vector<double> r;
+  r.push_back(detA);
+  r.push_back(detB);
+  r.push_back(wd.U);
+  r.push_back(wd.localEnergy);
+  r.push_back(wd.kineticEnergy);
+  r.push_back(wd.potentialEnergy);
+  r.push_back(wd.neEnergy);
+  r.push_back(wd.eeEnergy);
+  r.push_back(wd.singular ? 1.0 : 0.0);
+  r.push_back(wd.modificationRatio);
+  r.push_back(std::log(std::fabs(detA)) + std::log(std::fabs(detB)) + wd.U);
+  for (int e = 0; e < N; e++)
+    for (int k = 0; k < 3; k++) r.push_back(wd.gradPsiRatio(e, k));
+  for (int e = 0; e < N; e++)
+    for (int k = 0; k < 3; k++) r.push_back(wd.modifiedGradPsiRatio(e, k));
+  std::fwrite(&r[0], sizeof(double), r.size(), f);
+}
+
+static double now() {
+  struct timeval tv;
+  gettimeofday(&tv, 0);
+  return tv.tv_sec + 1e-6 * tv.tv_usec;
+}
+
+// splitmix64 + Box-Muller: the driver's own move generator (the reference's
+// ran1/gasdev streams are not part of the evaluate path).
+static unsigned long long g_state;
+static double unif() {
+  unsigned long long z = (g_state += 0x9E3779B97F4A7C15ULL);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  z ^= z >> 31;
+  return (z >> 11) * (1.0 / 9007199254740992.0);
+}
+static double gauss() {
+  double u1 = unif(), u2 = unif();
+  if (u1 < 1e-300) u1 = 1e-300;
+  return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+}
+
+int main(int argc, char **argv) {
+  if (argc < 5) {
+    std::fprintf(stderr, "usage: %s eval|seq|vmc input.ckmf in out|n_moves [tau seed]\n", argv[0]);
+    return 2;
+  }
+  std::string mode = argv[1];
+  globalInput.read(std::string(argv[2]));
+  QMCSCFJastrow qmf;
+  qmf.initialize(&globalInput, 0);
+  int N = globalInput.WF.getNumberElectrons();
+
+  std::FILE *fin = std::fopen(argv[3], "rb");
+  if (!fin) { std::perror("input"); return 1; }
+  if (mode == "eval") {
+    int hdr[2];
+    if (std::fread(hdr, sizeof(int), 2, fin) != 2 || hdr[1] != N) return 3;
+    std::FILE *fo = std::fopen(argv[4], "wb");
+    for (int c = 0; c < hdr[0]; c++) {
+      Array2D<double> R(N, 3);
+      for (int e = 0; e < N; e++)
+        for (int k = 0; k < 3; k++) {
+          double v;
+          if (std::fread(&v, sizeof(double), 1, fin) != 1) return 4;
+          R(e, k) = v;
+        }
+      QMCWalkerData wd;
+      wd.initialize(3, 0, 0);
+      wd.whichE = -1;
+      wd.updateDistances(R);
+      qmf.evaluate(R, wd);
+      record(fo, wd, N);
+    }
+    std::fclose(fo);
+  } else if (mode == "seq") {
+    int hdr[3];
+    if (std::fread(hdr, sizeof(int), 3, fin) != 3 || hdr[1] != N) return 3;
+    std::FILE *fo = std::fopen(argv[4], "wb");
+    for (int w = 0; w < hdr[0]; w++) {
+      Array2D<double> R(N, 3);
+      for (int e = 0; e < N; e++)
+        for (int k = 0; k < 3; k++) {
+          double v;
+          if (std::fread(&v, sizeof(double), 1, fin) != 1) return 4;
+          R(e, k) = v;
+        }
+      QMCWalkerData wd;
+      wd.initialize(3, 0, 0);
+      wd.whichE = -1;
+      wd.updateDistances(R);
+      qmf.evaluate(R, wd);
+      record(fo, wd, N);
+      for (int m = 0; m < hdr[2]; m++) {
+        double mv[4];
+        if (std::fread(mv, sizeof(double), 4, fin) != 4) return 4;
+        int e = (int)mv[0];
+        for (int k = 0; k < 3; k++) R(e, k) = mv[1 + k];
+        wd.whichE = e;
+        wd.updateDistances(R);
+        qmf.evaluate(R, wd);
+        record(fo, wd, N);
+      }
+    }
+    std::fclose(fo);
+  } else if (mode == "vmc") {
+    // Importance-sampled single-electron VMC with the reference evaluate:
+    // propose (QMCWalker.cpp:343-398), evaluate at the proposal, Metropolis
+    // with the Green's function ratio (:756-780, 1578-1685), and on rejection
+    // re-evaluate at the old position (:254-266).
+    int hdr[2];
+    if (std::fread(hdr, sizeof(int), 2, fin) != 2 || hdr[1] != N) return 3;
+    int nw = hdr[0];
+    int n_moves = std::atoi(argv[4]);
+    double tau = argc > 5 ? std::atof(argv[5]) : 0.01;
+    g_state = argc > 6 ? std::strtoull(argv[6], 0, 10) : 1;
+    std::vector<Array2D<double> > R(nw);
+    std::vector<QMCWalkerData *> wd(nw);
+    std::vector<double> lp(nw), el(nw);
+    std::vector<Array2D<double> > F(nw);
+    for (int w = 0; w < nw; w++) {
+      R[w].allocate(N, 3);
+      for (int e = 0; e < N; e++)
+        for (int k = 0; k < 3; k++) {
+          double v;
+          if (std::fread(&v, sizeof(double), 1, fin) != 1) return 4;
+          R[w](e, k) = v;
+        }
+      wd[w] = new QMCWalkerData();
+      wd[w]->initialize(3, 0, 0);
+      wd[w]->whichE = -1;
+      wd[w]->updateDistances(R[w]);
+      qmf.evaluate(R[w], *wd[w]);
+      lp[w] = std::log(std::fabs(wd[w]->DcA(0))) + std::log(std::fabs(wd[w]->DcB(0))) + wd[w]->U;
+      el[w] = wd[w]->localEnergy;
+      F[w] = wd[w]->modifiedGradPsiRatio;
+    }
+    double t0 = now(), esum = 0.0, acc = 0.0;
+    double sig = std::sqrt(tau);
+    for (int m = 0; m < n_moves; m++) {
+      int e = m % N;
+      for (int w = 0; w < nw; w++) {
+        double old[3], fwd = 0.0;
+        for (int i = 0; i < N; i++)
+          if (i != e)
+            for (int k = 0; k < 3; k++) fwd += tau * F[w](i, k) * tau * F[w](i, k);
+        for (int k = 0; k < 3; k++) {
+          old[k] = R[w](e, k);
+          double g = sig * gauss();
+          R[w](e, k) += tau * F[w](e, k) + g;
+          fwd += g * g;
+        }
+        wd[w]->whichE = e;
+        wd[w]->updateDistances(R[w]);
+        qmf.evaluate(R[w], *wd[w]);
+        double lpn = std::log(std::fabs(wd[w]->DcA(0))) + std::log(std::fabs(wd[w]->DcB(0))) + wd[w]->U;
+        double rev = 0.0;
+        for (int i = 0; i < N; i++)
+          for (int k = 0; k < 3; k++) {
+            double oc = i == e ? old[k] : R[w](i, k);
+            double t = oc - R[w](i, k) - tau * wd[w]->modifiedGradPsiRatio(i, k);
+            rev += t * t;
+          }
+        double p = wd[w]->singular ? 0.0 : std::exp(2.0 * (lpn - lp[w]) - (rev - fwd) / (2.0 * tau));
+        if (p > unif()) {
+          lp[w] = lpn; el[w] = wd[w]->localEnergy; F[w] = wd[w]->modifiedGradPsiRatio;
+          acc += 1.0;
+        } else {
+          for (int k = 0; k < 3; k++) R[w](e, k) = old[k];
+          wd[w]->updateDistances(R[w]);
+          qmf.evaluate(R[w], *wd[w]);
+        }
+        esum += el[w];
+      }
+    }
+    double dt = now() - t0;
+    std::printf("{\"seconds\": %.6f, \"walker_moves\": %lld, \"mean_e_local\": %.10f, \"acceptance\": %.6f}\n",
+                dt, (long long)nw * n_moves, esum / ((double)nw * n_moves), acc / ((double)nw * n_moves));
+  } else {
+    return 2;
+  }
+  std::fclose(fin);
+  return 0;
+}
