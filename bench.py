@@ -1,0 +1,217 @@
+"""Walker-moves/s of the MI355X walker-evaluation path.
+
+Workload (default): the 256-electron synthetic molecule (128 up + 128 down,
+40 nuclei, 1200 Cartesian s/p/d basis functions), 8192 walkers resident in
+HBM, single-electron importance-sampled VMC moves.  One step = every walker
+proposes a move of electron (step mod 256), the full evaluate contract runs at
+the proposal (ln|Psi|, sign, grad ln Psi of all electrons, Laplacian, E_L) and
+the move is accepted or rejected on the device.  The mixed path (basis + MFMA
+orbital GEMM in FP32, Sherman-Morrison inverse/log-det/E_L in FP64, FP64
+re-inversion every 64 moves) is `value`; the plain FP64 path on the same GPU
+is reported beside it.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "walker-moves/s at 256 e⁻ (mixed vs FP64, same GPU) + p50/p99 block latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def sm_bytes_per_move(n, N, s_d, s_o, acc):
+    """Algorithmic HBM bytes of one single-electron move in the sm_move kernel:
+    read A^-1 (8n^2) and the four derivative matrices (4 s_d n^2), the new
+    orbital rows (5 n s_o), current and proposed grad U (2 x 24N), both spins'
+    grad D/D (24N); on acceptance write A^-1 (8n^2), the new column of the five
+    stored matrices (5 n s_d), grad D/D of the moved spin (24n) and grad U (24N)."""
+    reads = 8 * n * n + 4 * s_d * n * n + 5 * n * s_o + 72 * N
+    writes = acc * (8 * n * n + 5 * n * s_d + 24 * n + 24 * N)
+    return reads + writes
+
+
+def run_path(s, precision, W, steps, warmup, tau, block, refresh, seed):
+    from qmcbeaver_amd import synth, Context
+    R0 = synth.initial_walkers(s, W, seed)
+    ctx = Context(s.pack(), precision=precision, max_walkers=W, refresh_interval=refresh,
+                  seed=seed)
+    ids = np.arange(W)
+    t0 = time.perf_counter()
+    res = ctx.evaluate(ids, R0)
+    t_init = time.perf_counter() - t0
+    n_sing = int(res["singular"].sum())
+    step = 0
+    if warmup:
+        ctx.vmc_steps(W, step, warmup, tau)
+        step += warmup
+    ctx.sync()
+    ctx.enable_timing(True)
+    lat = []
+    acc = 0.0
+    esum = 0.0
+    t0 = time.perf_counter()
+    done = 0
+    while done < steps:
+        k = min(block, steps - done)
+        tb = time.perf_counter()
+        st = ctx.vmc_steps(W, step, k, tau)
+        lat.append((time.perf_counter() - tb) * 1e3)
+        acc += st.sum_accept
+        esum += st.sum_e_local
+        step += k
+        done += k
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    kt = ctx.kernel_times()
+    ctx.enable_timing(False)
+    ctx.close()
+    return {
+        "walker_moves_per_s": W * steps / dt,
+        "ms_per_step": 1e3 * dt / steps,
+        "block_ms": lat,
+        "acceptance": acc / (W * steps),
+        "mean_e_local": esum / (W * steps),
+        "init_s": t_init,
+        "init_singular": n_sing,
+        "kernels": kt,
+    }
+
+
+def cpu_baseline(s, tau, seed, target_s=15.0):
+    """The reference's own evaluate (oracle/_ref, compiled from its sources)
+    timed on this host's cores, 1 thread; else the C oracle port."""
+    from qmcbeaver_amd import synth
+    drv = os.path.join(ROOT, "oracle", "_ref", "qmcb_ref_driver")
+    nw = 8
+    R = synth.initial_walkers(s, nw, seed)
+    if os.path.exists(drv):
+        with tempfile.TemporaryDirectory() as tmp:
+            ck = os.path.join(tmp, "in.ckmf")
+            synth.write_ckmf(s, ck, dt=tau)
+            cf = os.path.join(tmp, "cfg.bin")
+            with open(cf, "wb") as f:
+                np.array([nw, s.n_elec], dtype=np.int32).tofile(f)
+                R.tofile(f)
+            # calibrate on a short run, then a bounded sample of ~target_s
+            moves = 16
+            out = subprocess.run([drv, "vmc", ck, cf, str(moves), str(tau), "7"], cwd=tmp,
+                                 capture_output=True, text=True, check=True).stdout
+            r = json.loads(out.strip().splitlines()[-1])
+            rate = r["walker_moves"] / max(r["seconds"], 1e-9)
+            moves = int(max(16, min(4096, target_s * rate / nw)))
+            out = subprocess.run([drv, "vmc", ck, cf, str(moves), str(tau), "7"], cwd=tmp,
+                                 capture_output=True, text=True, check=True).stdout
+            r = json.loads(out.strip().splitlines()[-1])
+        return {"value": r["walker_moves"] / r["seconds"], "unit": "walker-moves/s", "cores": 1,
+                "kind": "reference",
+                "sample": f"{nw} walkers x {moves} single-electron VMC moves of the same molecule "
+                          f"(reference QMCSCFJastrow::evaluate, re-evaluate on rejection), "
+                          f"{r['seconds']:.1f} s"}
+    from oracle.pyoracle import Oracle
+    o = Oracle(s.pack())
+    t, _ = o.vmc_time(R, 32, tau)
+    rate = nw * 32 / t
+    moves = int(max(16, min(4096, target_s * rate / nw)))
+    t, _ = o.vmc_time(R, moves, tau)
+    return {"value": nw * moves / t, "unit": "walker-moves/s", "cores": 1, "kind": "port",
+            "sample": f"{nw} walkers x {moves} single-electron VMC moves (C oracle), {t:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=256)
+    ap.add_argument("--warmup", type=int, default=32)
+    ap.add_argument("--walkers", type=int, default=8192)
+    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--tau", type=float, default=0.01)
+    ap.add_argument("--block", type=int, default=16, help="steps per timed block")
+    ap.add_argument("--refresh", type=int, default=64)
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU: the walker-evaluation path "
+                                   "is measured at N=1 only (see DESIGN.md for how it shards)"}))
+        sys.exit(2)
+
+    from qmcbeaver_amd import synth
+    s = synth.make_system(args.config)
+    n, N = s.n_up, s.n_elec
+    mixed = run_path(s, args.precision, args.walkers, args.steps, args.warmup, args.tau,
+                     args.block, args.refresh if args.precision != "fp64" else 0, 1234)
+    fp64 = None
+    if not args.no_fp64 and args.precision != "fp64":
+        fp64 = run_path(s, "fp64", args.walkers, args.steps, args.warmup, args.tau, args.block,
+                        0, 1234)
+
+    s_d = 8 if args.precision == "fp64" else 4
+    kt = mixed["kernels"].get("sm_move", {"ms": float("nan"), "launches": 1})
+    avg_ms = kt["ms"] / max(kt["launches"], 1)
+    bytes_launch = args.walkers * sm_bytes_per_move(n, N, s_d, s_d, mixed["acceptance"])
+    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+    lat = np.array(mixed["block_ms"])
+    out = {
+        "metric": METRIC,
+        "value": mixed["walker_moves_per_s"],
+        "unit": "walker-moves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": mixed["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if args.precision != "fp64" else "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{s.name}: {N} e- ({n}+{s.n_dn}), {s.n_nuc} nuclei, {s.n_basis} "
+                        f"Cartesian s/p/d basis functions, {args.walkers} walkers, "
+                        f"single-electron importance-sampled VMC, tau={args.tau}",
+            "precision": "basis+orbital GEMM FP32 (MFMA), SM inverse/log-det/E_L FP64, "
+                         f"FP64 re-inversion every {args.refresh} moves"
+                         if args.precision != "fp64" else "FP64 throughout",
+            "walkers": args.walkers, "electrons": N, "basis_functions": s.n_basis,
+        },
+        "fp64": None if fp64 is None else {
+            "value": fp64["walker_moves_per_s"], "ms_per_step": fp64["ms_per_step"],
+            "acceptance": fp64["acceptance"], "mean_e_local": fp64["mean_e_local"],
+            "block_latency_ms": {"p50": float(np.percentile(fp64["block_ms"], 50)),
+                                 "p99": float(np.percentile(fp64["block_ms"], 99))}},
+        "mixed_over_fp64": None if fp64 is None else
+        mixed["walker_moves_per_s"] / fp64["walker_moves_per_s"],
+        "block_latency_ms": {"p50": float(np.percentile(lat, 50)),
+                             "p99": float(np.percentile(lat, 99)),
+                             "block_steps": args.block},
+        "acceptance": mixed["acceptance"],
+        "mean_e_local": mixed["mean_e_local"],
+        "roofline": {"kernel": "sm_move", "bound": "hbm", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "avg_launch_ms": avg_ms,
+                     "algorithmic_bytes_per_launch": bytes_launch},
+        "kernels_ms_per_step": {k: v["ms"] / max(args.steps, 1) for k, v in mixed["kernels"].items()},
+        "init_s": mixed["init_s"],
+    }
+    if fp64 is not None:
+        out["fp64"]["kernels_ms_per_step"] = {k: v["ms"] / max(args.steps, 1)
+                                              for k, v in fp64["kernels"].items()}
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(s, args.tau, 99)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,179 @@
+"""Python host binding of the C ABI (include/qmcbeaver_amd.h) via ctypes.
+
+The product path is the HIP library libqmcb_amd.so; this module only loads
+it and marshals arrays.  There is no CPU fallback: if the library is missing
+or cannot be loaded, `lib()` raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Dict, Optional, Sequence
+
+import numpy as np
+
+from ._abi import (Options, ResultArrays, Results, SystemBuffers, SystemStruct,
+                   VmcStats, PRECISIONS, STATUS, QMCB_DRIFT_NONE)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libqmcb_amd.so")
+
+EXPORTS = ["qmcb_abi_version", "qmcb_create", "qmcb_destroy", "qmcb_last_error",
+           "qmcb_evaluate", "qmcb_evaluate_device", "qmcb_accept", "qmcb_refresh",
+           "qmcb_get_positions", "qmcb_vmc_steps", "qmcb_sync", "qmcb_kernel_times",
+           "qmcb_enable_kernel_timing"]
+
+_lib = None
+
+
+class QmcbError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libqmcb_amd.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise QmcbError(f"{LIB_PATH} is missing: run __graft_entry__.build() "
+                        "(hipcc --offload-arch=gfx950) first")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    ip = C.POINTER(C.c_int32)
+    dp = C.POINTER(C.c_double)
+    L.qmcb_abi_version.restype = C.c_int32
+    L.qmcb_create.restype = C.c_int
+    L.qmcb_create.argtypes = [C.POINTER(SystemStruct), C.POINTER(Options), C.POINTER(vp)]
+    L.qmcb_destroy.argtypes = [vp]
+    L.qmcb_last_error.restype = C.c_char_p
+    L.qmcb_last_error.argtypes = [vp]
+    L.qmcb_evaluate.restype = C.c_int
+    L.qmcb_evaluate.argtypes = [vp, C.c_int32, ip, dp, C.c_int32, C.POINTER(Results)]
+    L.qmcb_evaluate_device.restype = C.c_int
+    L.qmcb_evaluate_device.argtypes = [vp, C.c_int32, vp, vp, C.c_int32, C.POINTER(Results)]
+    L.qmcb_accept.restype = C.c_int
+    L.qmcb_accept.argtypes = [vp, C.c_int32, ip, C.POINTER(C.c_uint8)]
+    L.qmcb_refresh.restype = C.c_int
+    L.qmcb_refresh.argtypes = [vp, C.c_int32, ip]
+    L.qmcb_get_positions.restype = C.c_int
+    L.qmcb_get_positions.argtypes = [vp, C.c_int32, dp]
+    L.qmcb_vmc_steps.restype = C.c_int
+    L.qmcb_vmc_steps.argtypes = [vp, C.c_int32, C.c_int64, C.c_int32, C.c_double,
+                                 C.POINTER(VmcStats)]
+    L.qmcb_sync.restype = C.c_int
+    L.qmcb_sync.argtypes = [vp]
+    L.qmcb_kernel_times.restype = C.c_int32
+    L.qmcb_kernel_times.argtypes = [vp, C.c_int32, C.POINTER(C.c_char_p), dp,
+                                    C.POINTER(C.c_int64)]
+    L.qmcb_enable_kernel_timing.argtypes = [vp, C.c_int32]
+    _lib = L
+    return L
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Context:
+    """One molecule + resident walkers on one GPU.
+
+    Mirrors QMCFunctions (QMCFunctions.h:26-105): `evaluate(ids, R, whichE)`
+    is QMCFunctions::evaluate(walkerData, xData, num) with the walker records
+    resident on the device; `accept` commits single-electron proposals.
+    """
+
+    def __init__(self, packed: Dict, *, precision: str = "fp64", max_walkers: int = 1,
+                 drift_mod: int = QMCB_DRIFT_NONE, dt: float = 0.1, umrigar_a: float = 0.5,
+                 refresh_interval: int = 0, seed: int = 12345, device: int = 0):
+        self._buf = SystemBuffers(packed)
+        self.N = self._buf.N
+        self.n_up = int(packed["n_up"])
+        self.max_walkers = max_walkers
+        o = Options()
+        o.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        o.max_walkers = max_walkers
+        o.drift_mod = drift_mod
+        o.refresh_interval = refresh_interval
+        o.dt = dt
+        o.umrigar_a = umrigar_a
+        o.seed = seed
+        o.device = device
+        self._opt = o
+        h = C.c_void_p()
+        st = lib().qmcb_create(C.byref(self._buf.struct), C.byref(o), C.byref(h))
+        if st != 0:
+            raise QmcbError(f"qmcb_create: {STATUS.get(st, st)}: "
+                            f"{lib().qmcb_last_error(None).decode()}")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().qmcb_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, st, what):
+        if st != 0:
+            raise QmcbError(f"{what}: {STATUS.get(st, st)}: "
+                            f"{lib().qmcb_last_error(self._h).decode()}")
+
+    def evaluate(self, ids: Sequence[int], R: np.ndarray, whichE: int = -1) -> ResultArrays:
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        R = np.ascontiguousarray(R, dtype=np.float64).reshape(len(ids), self.N, 3)
+        res = ResultArrays(len(ids), self.N)
+        self._check(lib().qmcb_evaluate(self._h, len(ids), _ip(ids), _dp(R), whichE,
+                                        C.byref(res.struct)), "qmcb_evaluate")
+        return res
+
+    def accept(self, ids: Sequence[int], mask: Sequence[bool]):
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        self._check(lib().qmcb_accept(self._h, len(ids), _ip(ids),
+                                      m.ctypes.data_as(C.POINTER(C.c_uint8))), "qmcb_accept")
+
+    def refresh(self, ids: Sequence[int]):
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        self._check(lib().qmcb_refresh(self._h, len(ids), _ip(ids)), "qmcb_refresh")
+
+    def positions(self, num: int) -> np.ndarray:
+        R = np.zeros((num, self.N, 3))
+        self._check(lib().qmcb_get_positions(self._h, num, _dp(R)), "qmcb_get_positions")
+        return R
+
+    def vmc_steps(self, num: int, first_step: int, n_steps: int, tau: float) -> VmcStats:
+        st = VmcStats()
+        self._check(lib().qmcb_vmc_steps(self._h, num, first_step, n_steps, tau,
+                                         C.byref(st)), "qmcb_vmc_steps")
+        return st
+
+    def sync(self):
+        self._check(lib().qmcb_sync(self._h), "qmcb_sync")
+
+    def enable_timing(self, on: bool = True):
+        lib().qmcb_enable_kernel_timing(self._h, 1 if on else 0)
+
+    def kernel_times(self) -> Dict[str, Dict[str, float]]:
+        cap = 32
+        names = (C.c_char_p * cap)()
+        ms = np.zeros(cap)
+        n = np.zeros(cap, dtype=np.int64)
+        k = lib().qmcb_kernel_times(self._h, cap, names, _dp(ms),
+                                    n.ctypes.data_as(C.POINTER(C.c_int64)))
+        return {names[i].decode(): {"ms": float(ms[i]), "launches": int(n[i])}
+                for i in range(min(k, cap))}

@@ -1,0 +1,348 @@
+// jastrow.hip -- kernel (d): two-body Pade Jastrow (e-e, e-n), Coulomb
+// potential and local-energy assembly, FP64.
+//
+// Reference: QMCJastrow::evaluate (QMCJastrow.cpp:134-187),
+// QMCJastrowElectronElectron::updateAll/updateOne/collectForPair
+// (QMCJastrowElectronElectron.cpp:56-298), QMCJastrowElectronNuclear::
+// evaluate (QMCJastrowElectronNuclear.cpp:54-187), QMCPotential_Energy::
+// calc_P_en/calc_P_ee (QMCPotential_Energy.cpp:242-333) and the assembly in
+// QMCSCFJastrow::calculate_Psi_quantities / calculate_E_Local
+// (QMCSCFJastrow.cpp:196-282, QMCSCFJastrow.h:311-314).
+//
+// Per pair: u, u' and u'' of the Pade function; U gets u, the gradient of
+// electron i gets u' (x_i - x_j)/r, the Laplacian gets 2(2u'/r + u'') per
+// e-e pair (both electrons) and 2u'/r + u'' per e-n pair.  The reference
+// caches every pair term in N x N arrays; here the single-electron update
+// recomputes the moved electron's old pair terms from its old position
+// instead (same values, no O(N^2) per-walker cache in HBM).
+#include "qmcb_kernels.hpp"
+
+namespace qmcb {
+
+__device__ __forceinline__ int ee_type(int i, int j, int n_up) {
+  const bool ui = i < n_up, uj = j < n_up;
+  return (ui != uj) ? 0 : (ui ? 1 : 2);
+}
+
+template <int NT>
+__device__ __forceinline__ void block_sum(double *vals, int nv, double *scratch /*[nv][NT/64]*/) {
+  constexpr int NW = NT / 64;
+  const int t = threadIdx.x;
+  for (int k = 0; k < nv; k++) {
+    const double s = wave_sum(vals[k]);
+    if ((t & 63) == 0) scratch[k * NW + (t >> 6)] = s;
+  }
+  __syncthreads();
+  for (int k = 0; k < nv; k++) {
+    double s = 0.0;
+    for (int q = 0; q < NW; q++) s += scratch[k * NW + q];
+    vals[k] = s;
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// all-electron Jastrow + potential, one workgroup per walker
+constexpr int kJT = 256;
+constexpr int kMaxN = 512;
+
+__global__ void __launch_bounds__(kJT) jastrow_all_kernel(JastrowArgs A, const int *walkers, int num) {
+  const int b = blockIdx.x;
+  if (b >= num) return;
+  const int w = walkers ? walkers[b] : b;
+  const DevSystem &S = A.sys;
+  const int N = S.N, nn = S.nn, nu = S.n[0];
+  __shared__ double x[kMaxN * 3];
+  __shared__ double scratch[4 * (kJT / 64)];
+  const double *Rw = A.R + (long)w * N * 3;
+  for (int i = threadIdx.x; i < 3 * N; i += kJT) x[i] = Rw[i];
+  __syncthreads();
+  double U = 0.0, Uxx = 0.0, Vee = 0.0, Ven = 0.0;
+  for (int i = threadIdx.x; i < N; i += kJT) {
+    double g[3] = {0.0, 0.0, 0.0};
+    const int sp = i < nu ? 0 : 1;
+    for (int A_ = 0; A_ < nn; A_++) {
+      double d[3], r2 = 0.0;
+      for (int k = 0; k < 3; k++) { d[k] = x[3 * i + k] - S.nuc_xyz[3 * A_ + k]; r2 += d[k] * d[k]; }
+      const double r = sqrt(r2);
+      double u, du, d2u;
+      pade_eval(S.en[sp][A_], r, u, du, d2u);
+      U += u;
+      Uxx += 2.0 / r * du + d2u;
+      for (int k = 0; k < 3; k++) g[k] += du * (d[k] / r);
+      Ven -= S.nuc_z[A_] / r;
+    }
+    for (int j = 0; j < N; j++) {
+      if (j == i) continue;
+      double d[3], r2 = 0.0;
+      for (int k = 0; k < 3; k++) { d[k] = x[3 * i + k] - x[3 * j + k]; r2 += d[k] * d[k]; }
+      const double r = sqrt(r2);
+      double u, du, d2u;
+      pade_eval(S.ee[ee_type(i, j, nu)], r, u, du, d2u);
+      for (int k = 0; k < 3; k++) g[k] += du * (d[k] / r);
+      if (j < i) {
+        U += u;
+        Uxx += 2.0 * (2.0 / r * du + d2u);
+        Vee += 1.0 / r;
+      }
+    }
+    double *ux = A.Ux + ((long)w * N + i) * 3;
+    ux[0] = g[0]; ux[1] = g[1]; ux[2] = g[2];
+  }
+  double v[4] = {U, Uxx, Vee, Ven};
+  block_sum<kJT>(v, 4, scratch);
+  if (threadIdx.x == 0) {
+    A.U[w] = v[0];
+    A.Uxx[w] = v[1];
+    A.Vee[w] = v[2];
+    A.Ven[w] = v[3];
+  }
+}
+
+hipError_t launch_jastrow_all(const JastrowArgs &a, const int *walkers, int num, hipStream_t st) {
+  if (num <= 0) return hipSuccess;
+  if (a.sys.N > kMaxN) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(jastrow_all_kernel, dim3(num), dim3(kJT), 0, st, a, walkers, num);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// single-electron Jastrow + potential at the proposal (updateOne,
+// QMCJastrowElectronElectron.cpp:68-152; e-n of the moved electron only,
+// QMCJastrowElectronNuclear.cpp:117-120; Coulomb difference of its pairs).
+// Writes U', grad U' for all N electrons, Lap U', V_ee', V_en' to batch row b.
+constexpr int kMT = 128;
+
+__global__ void __launch_bounds__(kMT)
+jastrow_move_kernel(JastrowArgs A, const double *cU, const double *cUx, const double *cUxx,
+                    const double *cVee, const double *cVen, const int *walkers, int num) {
+  const int b = blockIdx.x;
+  if (b >= num) return;
+  const int w = walkers ? walkers[b] : b;
+  const DevSystem &S = A.sys;
+  const int N = S.N, nn = S.nn, nu = S.n[0], m = A.moved;
+  __shared__ double scratch[7 * (kMT / 64)];
+  const double *Rw = A.R + (long)w * N * 3;
+  const double xo[3] = {Rw[3 * m], Rw[3 * m + 1], Rw[3 * m + 2]};
+  const double xn[3] = {A.propR[3L * b], A.propR[3L * b + 1], A.propR[3L * b + 2]};
+  const double *ux = cUx + (long)w * N * 3;
+  double *pux = A.Ux + (long)b * N * 3;
+  double dU = 0.0, dUxx = 0.0, dVee = 0.0, dVen = 0.0, gm[3] = {0.0, 0.0, 0.0};
+  for (int j = threadIdx.x; j < N; j += kMT) {
+    if (j == m) continue;
+    const double xj[3] = {Rw[3 * j], Rw[3 * j + 1], Rw[3 * j + 2]};
+    double dn[3], dold[3], rn2 = 0.0, ro2 = 0.0;
+    for (int k = 0; k < 3; k++) {
+      dn[k] = xn[k] - xj[k]; rn2 += dn[k] * dn[k];
+      dold[k] = xo[k] - xj[k]; ro2 += dold[k] * dold[k];
+    }
+    const double rn = sqrt(rn2), ro = sqrt(ro2);
+    const qmcb_pade &f = S.ee[ee_type(m, j, nu)];
+    double un, dun, d2un, uo, duo, d2uo;
+    pade_eval(f, rn, un, dun, d2un);
+    pade_eval(f, ro, uo, duo, d2uo);
+    dU += un - uo;
+    dUxx += 2.0 * (2.0 / rn * dun + d2un) - 2.0 * (2.0 / ro * duo + d2uo);
+    dVee += 1.0 / rn - 1.0 / ro;
+    for (int k = 0; k < 3; k++) {
+      // gradient on j: u'(r) (x_j - x_m)/r
+      pux[3 * j + k] = ux[3 * j + k] - dun * (dn[k] / rn) + duo * (dold[k] / ro);
+      gm[k] += dun * (dn[k] / rn);
+    }
+  }
+  const int sp = m < nu ? 0 : 1;
+  for (int a_ = threadIdx.x; a_ < nn; a_ += kMT) {
+    double dn[3], dold[3], rn2 = 0.0, ro2 = 0.0;
+    for (int k = 0; k < 3; k++) {
+      dn[k] = xn[k] - S.nuc_xyz[3 * a_ + k]; rn2 += dn[k] * dn[k];
+      dold[k] = xo[k] - S.nuc_xyz[3 * a_ + k]; ro2 += dold[k] * dold[k];
+    }
+    const double rn = sqrt(rn2), ro = sqrt(ro2);
+    double un, dun, d2un, uo, duo, d2uo;
+    pade_eval(S.en[sp][a_], rn, un, dun, d2un);
+    pade_eval(S.en[sp][a_], ro, uo, duo, d2uo);
+    dU += un - uo;
+    dUxx += (2.0 / rn * dun + d2un) - (2.0 / ro * duo + d2uo);
+    dVen += -S.nuc_z[a_] / rn + S.nuc_z[a_] / ro;
+    for (int k = 0; k < 3; k++) gm[k] += dun * (dn[k] / rn);
+  }
+  double v[7] = {dU, dUxx, dVee, dVen, gm[0], gm[1], gm[2]};
+  block_sum<kMT>(v, 7, scratch);
+  if (threadIdx.x == 0) {
+    pux[3 * m] = v[4];
+    pux[3 * m + 1] = v[5];
+    pux[3 * m + 2] = v[6];
+  }
+  if (threadIdx.x == 1) {
+    // outputs of the proposal, batch row b (A.U etc. point at the proposal arrays)
+    A.U[b] = cU[w] + v[0];
+    A.Uxx[b] = cUxx[w] + v[1];
+    A.Vee[b] = cVee[w] + v[2];
+    A.Ven[b] = cVen[w] + v[3];
+  }
+}
+
+hipError_t launch_jastrow_move(const JastrowArgs &a, const double *U, const double *Ux,
+                               const double *Uxx, const double *Vee, const double *Ven,
+                               const int *walkers, int num, hipStream_t st) {
+  if (num <= 0) return hipSuccess;
+  hipLaunchKernelGGL(jastrow_move_kernel, dim3(num), dim3(kMT), 0, st, a, U, Ux, Uxx, Vee, Ven,
+                     walkers, num);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Drift modification helper shared with the move kernel (defined in slater.hip)
+__device__ inline double drift_factor_unequal_j(const double g[3], const double x[3], int nn,
+                                                const double *nuc_xyz, const double *nuc_z,
+                                                double dt) {
+  int cn = 0;
+  double best = 1e300;
+  for (int A = 0; A < nn; A++) {
+    const double dx = x[0] - nuc_xyz[3 * A], dy = x[1] - nuc_xyz[3 * A + 1],
+                 dz = x[2] - nuc_xyz[3 * A + 2];
+    const double r = sqrt(dx * dx + dy * dy + dz * dz);
+    if (r < best) { best = r; cn = A; }
+  }
+  double nv[3], dsq = 0.0;
+  for (int k = 0; k < 3; k++) { nv[k] = x[k] - nuc_xyz[3 * cn + k]; dsq += nv[k] * nv[k]; }
+  const double inr = 1.0 / sqrt(dsq);
+  const double v2 = g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
+  if (fabs(v2) < 1e-10) return 1.0;
+  const double iv = 1.0 / sqrt(v2);
+  const double dotp = (nv[0] * g[0] + nv[1] * g[1] + nv[2] * g[2]) * inr * iv;
+  const double Z = nuc_z[cn];
+  const double t = Z * Z * dsq;
+  const double a = 0.5 * (1.0 + dotp) + t / (10.0 * (4.0 + t));
+  return (-1.0 + sqrt(1.0 + 2.0 * a * v2 * dt)) / (a * v2 * dt);
+}
+
+// all-electron assembly, one workgroup per walker
+__global__ void __launch_bounds__(kJT) assemble_kernel(AssembleArgs A, const int *walkers, int num) {
+  const int b = blockIdx.x;
+  if (b >= num) return;
+  const int w = walkers ? walkers[b] : b;
+  const DevSystem &S = A.sys;
+  const int N = S.N, nu = S.n[0];
+  __shared__ double scratch[2 * (kJT / 64)];
+  const double *ux = A.Ux + (long)w * N * 3;
+  const double *Rw = A.R + (long)w * N * 3;
+  double lsum = 0.0, gsq = 0.0;
+  for (int e = threadIdx.x; e < N; e += kJT) {
+    const int sp = e < nu ? 0 : 1;
+    const int le = e - S.start[sp];
+    const double *gp = A.s[sp].gradPR + ((long)w * S.n[sp] + le) * 3;
+    double g[3];
+    for (int k = 0; k < 3; k++) {
+      g[k] = gp[k] + ux[3 * e + k];
+      lsum += ux[3 * e + k] * gp[k] * 2.0;
+      lsum += ux[3 * e + k] * ux[3 * e + k];
+      gsq += g[k] * g[k];
+    }
+    if (A.out.grad_lnpsi) {
+      double *o = A.out.grad_lnpsi + ((long)b * N + e) * 3;
+      o[0] = g[0]; o[1] = g[1]; o[2] = g[2];
+    }
+  }
+  double v[2] = {lsum, gsq};
+  block_sum<kJT>(v, 2, scratch);
+  bool sing = false;
+  double logdet = 0.0, sgn = 1.0, lapR = A.Uxx[w] + v[0];
+  for (int sp = 0; sp < 2; sp++) {
+    if (S.n[sp] == 0) continue;
+    sing = sing || A.s[sp].singular[w];
+    logdet += A.s[sp].logdet[w];
+    sgn *= A.s[sp].sgn[w];
+    lapR += A.s[sp].lapPR[w];
+  }
+  if (!(v[1] > 0.0) || !isfinite(v[1])) sing = true;
+  if (A.out.mod_drift) {
+    double feq = 1.0;
+    if (A.drift_mod == QMCB_DRIFT_UMRIGAR93_EQUAL)
+      feq = (-1.0 + sqrt(1.0 + 2 * A.umrigar_a * v[1] * A.dt)) / (A.umrigar_a * v[1] * A.dt);
+    for (int e = threadIdx.x; e < N; e += kJT) {
+      const int sp = e < nu ? 0 : 1;
+      const int le = e - S.start[sp];
+      const double *gp = A.s[sp].gradPR + ((long)w * S.n[sp] + le) * 3;
+      double g[3];
+      for (int k = 0; k < 3; k++) g[k] = gp[k] + ux[3 * e + k];
+      double f = feq;
+      if (A.drift_mod == QMCB_DRIFT_UMRIGAR93_UNEQUAL) {
+        const double x[3] = {Rw[3 * e], Rw[3 * e + 1], Rw[3 * e + 2]};
+        f = drift_factor_unequal_j(g, x, S.nn, S.nuc_xyz, S.nuc_z, A.dt);
+      }
+      double *o = A.out.mod_drift + ((long)b * N + e) * 3;
+      o[0] = g[0] * f; o[1] = g[1] * f; o[2] = g[2] * f;
+    }
+  }
+  if (threadIdx.x == 0) {
+    const double V = S.v_nn + A.Vee[w] + A.Ven[w];
+    const double EL = -0.5 * lapR + V;
+    const qmcb_results &o = A.out;
+    if (o.log_psi) o.log_psi[b] = logdet + A.U[w];
+    if (o.sign_psi) o.sign_psi[b] = (int8_t)(sgn < 0 ? -1 : 1);
+    if (o.lap_ratio) o.lap_ratio[b] = lapR;
+    if (o.e_local) o.e_local[b] = EL;
+    if (o.kinetic) o.kinetic[b] = -0.5 * lapR;
+    if (o.potential) o.potential[b] = V;
+    if (o.v_ne) o.v_ne[b] = A.Ven[w];
+    if (o.v_ee) o.v_ee[b] = A.Vee[w];
+    if (o.det_ratio) o.det_ratio[b] = 1.0;
+    if (o.jastrow_u) o.jastrow_u[b] = A.U[w];
+    if (o.singular) o.singular[b] = sing ? 1 : 0;
+    if (A.cur_el) A.cur_el[w] = EL;
+  }
+}
+
+hipError_t launch_assemble(const AssembleArgs &a, const int *walkers, int num, hipStream_t st) {
+  if (num <= 0) return hipSuccess;
+  hipLaunchKernelGGL(assemble_kernel, dim3(num), dim3(kJT), 0, st, a, walkers, num);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Importance-sampled proposal of electron `moved` (QMCWalker.cpp:343-398):
+//   x' = x + tau F + sqrt(tau) g,  g ~ N(0,1)^3,  F = grad ln Psi (drift)
+// Random numbers: Philox4x32-10 keyed by (seed), counter (step, walker),
+// so every walker's stream is independent of the batch it runs in.
+__global__ void __launch_bounds__(256) propose_kernel(ProposeArgs A, const int *walkers, int num) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= num) return;
+  const int w = walkers ? walkers[b] : b;
+  const int m = A.moved;
+  const int sp = m < A.n_up ? 0 : 1;
+  const int le = sp == 0 ? m : m - A.n_up;
+  const int nsp = sp == 0 ? A.n_up : A.N - A.n_up;
+  const double *gp = A.gradPR[sp] + ((long)w * nsp + le) * 3;
+  const double *ux = A.Ux + ((long)w * A.N + m) * 3;
+  const double *x = A.R + ((long)w * A.N + m) * 3;
+  const uint32_t k0 = (uint32_t)A.seed, k1 = (uint32_t)(A.seed >> 32);
+  double u[8];
+  for (int c = 0; c < 2; c++) {
+    u4 ctr = {(uint32_t)A.step, (uint32_t)((uint64_t)A.step >> 32), (uint32_t)w, (uint32_t)c};
+    u4 r = philox4x32(ctr, k0, k1);
+    u[4 * c + 0] = u01(r.x, r.y);
+    u[4 * c + 1] = u01(r.y, r.z);
+    u[4 * c + 2] = u01(r.z, r.w);
+    u[4 * c + 3] = u01(r.w, r.x);
+  }
+  // Box-Muller on (u0,u1), (u2,u3)
+  const double two_pi = 6.283185307179586;
+  const double r1 = sqrt(-2.0 * log(u[0])), r2 = sqrt(-2.0 * log(u[2]));
+  const double g[3] = {r1 * cos(two_pi * u[1]), r1 * sin(two_pi * u[1]), r2 * cos(two_pi * u[3])};
+  const double st = sqrt(A.tau);
+  for (int k = 0; k < 3; k++) {
+    const double chi = st * g[k];
+    A.gauss[3L * b + k] = chi;
+    A.propR[3L * b + k] = x[k] + A.tau * (gp[k] + ux[k]) + chi;
+  }
+  A.unif[b] = u[4];
+}
+
+hipError_t launch_propose(const ProposeArgs &a, const int *walkers, int num, hipStream_t st) {
+  if (num <= 0) return hipSuccess;
+  hipLaunchKernelGGL(propose_kernel, dim3((num + 255) / 256), dim3(256), 0, st, a, walkers, num);
+  return hipGetLastError();
+}
+
+}  // namespace qmcb

@@ -1,0 +1,121 @@
+// orbital_gemm.hip -- kernel (b): the orbital contraction on MFMA.
+//
+// Reference: QMCSlater::evaluate's five products Xw * C^T (QMCSlater.cpp:
+// 856-862, Array2D::gemm with rhsIsTransposed, Array2D.h:425-504).  All five
+// components (value, d/dx, d/dy, d/dz, Laplacian) and every point of a batch
+// form one GEMM, computed transposed so both operands stream coalesced:
+//
+//   D^T[k][o][p] = sum_b  Ct[b][o] * X[k][b][p]
+//
+// Ct = C^T ([basis][orbital], zero padded to 16-multiples of orbitals and
+// 4-multiples of basis functions), X from kernel (a).  One wavefront owns a
+// 16-point tile and MT 16-orbital tiles; each k-step issues MT MFMA
+// 16x16x4 instructions (FP64: v_mfma_f64_16x16x4f64, FP32:
+// v_mfma_f32_16x16x4f32) with lane l supplying A[l%16][l/16] and
+// B[l/16][l%16]; lane l receives D[4(l/16)+i][l%16], i = 0..3.
+#include "qmcb_kernels.hpp"
+
+namespace qmcb {
+
+typedef double d4v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef short s4v __attribute__((ext_vector_type(4)));
+
+template <typename TI>
+struct Mfma;
+
+template <>
+struct Mfma<double> {
+  typedef d4v acc_t;
+  typedef double acc_s;
+  static constexpr int KSTEP = 4;
+  __device__ static acc_t run(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+
+template <>
+struct Mfma<float> {
+  typedef f4v acc_t;
+  typedef float acc_s;
+  static constexpr int KSTEP = 4;
+  __device__ static acc_t run(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+template <typename TI, typename TO, int MT>
+__global__ void __launch_bounds__(256)
+orbital_gemm_kernel(const TI *__restrict__ Ct, int ldc, int n_orb, const TI *__restrict__ X,
+                    int ldk, long ldp, long P, int kpad, GemmOut o) {
+  typedef Mfma<TI> M;
+  typedef typename M::acc_t acc_t;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long p0 = ((long)blockIdx.x * 4 + wave) * 16;
+  if (p0 >= P) return;  // wave-uniform
+  const int comp = blockIdx.y;
+  const int o0 = blockIdx.z * 16 * MT;
+  const int li = lane & 15, lk = lane >> 4;
+  const TI *__restrict__ xp = X + (long)comp * ldk * ldp + (long)lk * ldp + p0 + li;
+  const TI *__restrict__ cp = Ct + (long)lk * ldc + o0 + li;
+  acc_t acc[MT];
+#pragma unroll
+  for (int t = 0; t < MT; t++) acc[t] = acc_t{0, 0, 0, 0};
+
+  for (int k0 = 0; k0 < kpad; k0 += 4) {
+    const TI b = xp[(long)k0 * ldp];
+    TI a[MT];
+#pragma unroll
+    for (int t = 0; t < MT; t++) a[t] = cp[(long)k0 * ldc + 16 * t];
+#pragma unroll
+    for (int t = 0; t < MT; t++) acc[t] = M::run(a[t], b, acc[t]);
+  }
+  const long p = p0 + li;
+  if (p >= P) return;
+  TO *out = (TO *)o.out;
+  const long row = p / o.per_walker;
+  const long wk = o.walker_map ? o.walker_map[row] : row;
+  const long base = (long)comp * o.s_comp + wk * o.s_walker + (p % o.per_walker) * o.s_elec;
+#pragma unroll
+  for (int t = 0; t < MT; t++) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int orb = o0 + 16 * t + 4 * lk + i;
+      if (orb < n_orb) out[base + (long)orb * o.s_orb] = (TO)acc[t][i];
+    }
+  }
+}
+
+template <typename TI, typename TO>
+hipError_t launch_orbital_gemm(const TI *Ct, int ldc, int n_orb, const TI *X, int ldk, long ldp,
+                               long P, int kpad, const GemmOut &o, hipStream_t st) {
+  if (P <= 0 || n_orb <= 0) return hipSuccess;
+  const unsigned gx = (unsigned)((P + 63) / 64);
+  if (n_orb > 64) {
+    dim3 grid(gx, kComp, (unsigned)((n_orb + 127) / 128));
+    hipLaunchKernelGGL((orbital_gemm_kernel<TI, TO, 8>), grid, dim3(256), 0, st, Ct, ldc, n_orb,
+                       X, ldk, ldp, P, kpad, o);
+  } else if (n_orb > 32) {
+    dim3 grid(gx, kComp, 1);
+    hipLaunchKernelGGL((orbital_gemm_kernel<TI, TO, 4>), grid, dim3(256), 0, st, Ct, ldc, n_orb,
+                       X, ldk, ldp, P, kpad, o);
+  } else if (n_orb > 16) {
+    dim3 grid(gx, kComp, 1);
+    hipLaunchKernelGGL((orbital_gemm_kernel<TI, TO, 2>), grid, dim3(256), 0, st, Ct, ldc, n_orb,
+                       X, ldk, ldp, P, kpad, o);
+  } else {
+    dim3 grid(gx, kComp, 1);
+    hipLaunchKernelGGL((orbital_gemm_kernel<TI, TO, 1>), grid, dim3(256), 0, st, Ct, ldc, n_orb,
+                       X, ldk, ldp, P, kpad, o);
+  }
+  return hipGetLastError();
+}
+
+template hipError_t launch_orbital_gemm<double, double>(const double *, int, int, const double *,
+                                                        int, long, long, int, const GemmOut &,
+                                                        hipStream_t);
+template hipError_t launch_orbital_gemm<float, float>(const float *, int, int, const float *, int,
+                                                      long, long, int, const GemmOut &,
+                                                      hipStream_t);
+
+}  // namespace qmcb

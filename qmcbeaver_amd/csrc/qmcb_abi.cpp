@@ -1,0 +1,858 @@
+// qmcb_abi.cpp -- the C ABI (include/qmcbeaver_amd.h) over the HIP kernels.
+//
+// A context owns one molecule and the device-resident state of up to
+// max_walkers walkers.  Every entry point runs on the context's stream; the
+// host-buffer entry points copy in, launch, copy out and synchronise.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "qmcb_kernels.hpp"
+
+using namespace qmcb;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct Timing {
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  std::vector<std::string> names;
+  std::map<std::string, int> index;
+  std::vector<double> ms;
+  std::vector<long long> launches;
+};
+
+}  // namespace
+
+struct qmcb_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  qmcb_options opt{};
+  int N = 0, n[2] = {0, 0}, start[2] = {0, 0}, nn = 0, B = 0, kpad = 0, nshell = 0, nprim = 0;
+  int npad[2] = {16, 16};
+  int Wc = 0;
+  DevSystem dsys{};
+  std::vector<void *> allocs;
+  size_t ts = 8, tx = 8, to = 8;  // bytes: derivative storage, GEMM operand, GEMM output
+
+  // resident walker state
+  double *R = nullptr;
+  double *inv[2] = {nullptr, nullptr};
+  void *dmat[2] = {nullptr, nullptr};
+  double *logdet[2] = {nullptr, nullptr}, *sgn[2] = {nullptr, nullptr};
+  double *gradPR[2] = {nullptr, nullptr}, *lapPR[2] = {nullptr, nullptr};
+  int *singular[2] = {nullptr, nullptr};
+  double *U = nullptr, *Ux = nullptr, *Uxx = nullptr, *Vee = nullptr, *Ven = nullptr;
+  double *cur_el = nullptr;
+  // batch-row staging
+  double *Rin = nullptr;
+  double *propR = nullptr, *gauss = nullptr, *unif = nullptr;
+  double *pU = nullptr, *pUx = nullptr, *pUxx = nullptr, *pVee = nullptr, *pVen = nullptr;
+  void *vrow = nullptr;
+  long ldv = 0;
+  double *st_inv = nullptr, *st_grad = nullptr, *st_lap = nullptr, *st_logdet = nullptr,
+         *st_sgn = nullptr;
+  int *st_singular = nullptr;
+  // basis scratch
+  void *X = nullptr;
+  long ldp = 0;      // point capacity (multiple of 64)
+  // coefficients, transposed [kpad][npad] per spin, in the GEMM operand type
+  void *Ct[2] = {nullptr, nullptr};
+  // device output arrays (batch rows) for the host-buffer entry points
+  qmcb_results dout{};
+  int *d_ids = nullptr;
+  uint8_t *d_mask = nullptr;
+  // vmc accumulators
+  double *acc_el = nullptr, *acc_el2 = nullptr, *acc_n = nullptr, *acc_prop_el = nullptr;
+  // staged single-electron pass
+  int staged_moved = -1;
+  int staged_num = 0;
+  std::vector<int> staged_ids;
+  std::map<int, int> staged_row;
+  std::vector<int> initialized;  // per slot: has a resident state
+  // timing
+  bool timing = false;
+  Timing tm;
+  std::string err;
+};
+
+namespace {
+
+#define QCHK(expr)                                                              \
+  do {                                                                          \
+    hipError_t _e = (expr);                                                     \
+    if (_e != hipSuccess) {                                                     \
+      ctx->err = std::string(#expr) + ": " + hipGetErrorString(_e);             \
+      return QMCB_DEVICE_ERROR;                                                 \
+    }                                                                           \
+  } while (0)
+
+template <typename T>
+hipError_t dalloc(qmcb_ctx *ctx, T **p, size_t count) {
+  void *q = nullptr;
+  size_t bytes = std::max<size_t>(count * sizeof(T), 16);
+  hipError_t e = hipMalloc(&q, bytes);
+  if (e != hipSuccess) return e;
+  e = hipMemset(q, 0, bytes);
+  ctx->allocs.push_back(q);
+  *p = (T *)q;
+  return e;
+}
+
+hipError_t dalloc_bytes(qmcb_ctx *ctx, void **p, size_t bytes) {
+  bytes = std::max<size_t>(bytes, 16);
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) return e;
+  e = hipMemset(*p, 0, bytes);
+  ctx->allocs.push_back(*p);
+  return e;
+}
+
+int pad_orb(int n) {
+  if (n > 64) return ((n + 127) / 128) * 128;
+  if (n > 32) return 64;
+  if (n > 16) return 32;
+  return 16;
+}
+
+// ---- timing helpers --------------------------------------------------------
+void tm_begin(qmcb_ctx *ctx, const char *name, hipEvent_t *e0) {
+  if (!ctx->timing) return;
+  Timing &t = ctx->tm;
+  if (t.used + 2 > t.pool.size()) {
+    for (int i = 0; i < 64; i++) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return;
+      t.pool.push_back(e);
+    }
+  }
+  *e0 = t.pool[t.used];
+  (void)hipEventRecord(*e0, ctx->stream);
+  (void)name;
+}
+void tm_end(qmcb_ctx *ctx, const char *name, hipEvent_t e0) {
+  if (!ctx->timing) return;
+  Timing &t = ctx->tm;
+  hipEvent_t e1 = t.pool[t.used + 1];
+  (void)hipEventRecord(e1, ctx->stream);
+  auto it = t.index.find(name);
+  int id;
+  if (it == t.index.end()) {
+    id = (int)t.names.size();
+    t.index[name] = id;
+    t.names.push_back(name);
+    t.ms.push_back(0.0);
+    t.launches.push_back(0);
+  } else {
+    id = it->second;
+  }
+  t.marks.push_back({id, {e0, e1}});
+  t.used += 2;
+}
+void tm_collect(qmcb_ctx *ctx) {
+  if (!ctx->timing) return;
+  Timing &t = ctx->tm;
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto &m : t.marks) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, m.second.first, m.second.second);
+    t.ms[m.first] += ms;
+    t.launches[m.first] += 1;
+  }
+  t.marks.clear();
+  t.used = 0;
+}
+
+#define TIMED(name, call)               \
+  do {                                  \
+    hipEvent_t _e0 = nullptr;           \
+    tm_begin(ctx, name, &_e0);          \
+    hipError_t _r = (call);             \
+    tm_end(ctx, name, _e0);             \
+    if (_r != hipSuccess) {             \
+      ctx->err = std::string(name) + ": " + hipGetErrorString(_r); \
+      return QMCB_DEVICE_ERROR;         \
+    }                                   \
+  } while (0)
+
+SlaterArgs slater_args(qmcb_ctx *ctx, int sp) {
+  SlaterArgs a{};
+  a.n = ctx->n[sp];
+  a.N = ctx->N;
+  a.spin_start = ctx->start[sp];
+  a.inv = ctx->inv[sp];
+  a.dmat = ctx->dmat[sp];
+  a.logdet = ctx->logdet[sp];
+  a.sgn = ctx->sgn[sp];
+  a.gradPR = ctx->gradPR[sp];
+  a.lapPR = ctx->lapPR[sp];
+  a.singular = ctx->singular[sp];
+  return a;
+}
+
+// ---- precision dispatch ------------------------------------------------------
+// TS: derivative storage, TX: basis/GEMM operand, TO: GEMM output
+template <typename TS, typename TX, typename TC, typename TO>
+struct Engine {
+  // all-electron: slots given by d_map (device) for batch rows [0, num)
+  static qmcb_status all_electron(qmcb_ctx *ctx, const int *d_map, int num) {
+    for (int sp = 0; sp < 2; sp++) {
+      const int n = ctx->n[sp];
+      if (n == 0) continue;
+      const int chunk = std::max<long>(1, std::min<long>(num, ctx->ldp / n));
+      for (int c0 = 0; c0 < num; c0 += chunk) {
+        const int cw = std::min(chunk, num - c0);
+        PointSet pts{ctx->R + 3L * ctx->start[sp], (long)cw * n, n, 3L * ctx->N, 3,
+                     d_map + c0};
+        TIMED("basis", (launch_basis<TC, TX>(ctx->dsys, pts, (TX *)ctx->X, ctx->kpad, ctx->ldp,
+                                            ctx->stream)));
+        GemmOut go{ctx->dmat[sp], (long)n * n, (long)n, (long)kComp * n * n, 1, n, d_map + c0};
+        TIMED("orbital_gemm", (launch_orbital_gemm<TX, TS>((const TX *)ctx->Ct[sp], ctx->npad[sp], n,
+                                                          (const TX *)ctx->X, ctx->kpad, ctx->ldp,
+                                                          (long)cw * n, ctx->kpad, go,
+                                                          ctx->stream)));
+      }
+      SlaterArgs sa = slater_args(ctx, sp);
+      TIMED("lu_inverse", launch_lu_inverse<TS>(sa, d_map, num, ctx->stream));
+      TIMED("traces_all", launch_traces_all<TS>(sa, d_map, num, ctx->stream));
+    }
+    JastrowArgs ja{};
+    ja.sys = ctx->dsys;
+    ja.R = ctx->R;
+    ja.U = ctx->U; ja.Ux = ctx->Ux; ja.Uxx = ctx->Uxx; ja.Vee = ctx->Vee; ja.Ven = ctx->Ven;
+    TIMED("jastrow_all", launch_jastrow_all(ja, d_map, num, ctx->stream));
+    return QMCB_OK;
+  }
+
+  static qmcb_status assemble(qmcb_ctx *ctx, const int *d_map, int num, const qmcb_results &out) {
+    AssembleArgs aa{};
+    aa.sys = ctx->dsys;
+    aa.s[0] = slater_args(ctx, 0);
+    aa.s[1] = slater_args(ctx, 1);
+    aa.R = ctx->R; aa.U = ctx->U; aa.Ux = ctx->Ux; aa.Uxx = ctx->Uxx;
+    aa.Vee = ctx->Vee; aa.Ven = ctx->Ven;
+    aa.drift_mod = ctx->opt.drift_mod;
+    aa.dt = ctx->opt.dt;
+    aa.umrigar_a = ctx->opt.umrigar_a;
+    aa.out = out;
+    aa.cur_el = ctx->cur_el;
+    TIMED("assemble", launch_assemble(aa, d_map, num, ctx->stream));
+    return QMCB_OK;
+  }
+
+  // proposal rows: propR[b] set; Jastrow at the proposal, basis + GEMM row
+  static qmcb_status move_front(qmcb_ctx *ctx, const int *d_map, int num, int m) {
+    const int sp = m < ctx->n[0] ? 0 : 1;
+    JastrowArgs ja{};
+    ja.sys = ctx->dsys;
+    ja.R = ctx->R;
+    ja.U = ctx->pU; ja.Ux = ctx->pUx; ja.Uxx = ctx->pUxx; ja.Vee = ctx->pVee; ja.Ven = ctx->pVen;
+    ja.moved = m;
+    ja.propR = ctx->propR;
+    TIMED("jastrow_move", launch_jastrow_move(ja, ctx->U, ctx->Ux, ctx->Uxx, ctx->Vee, ctx->Ven,
+                                              d_map, num, ctx->stream));
+    PointSet pts{ctx->propR, (long)num, 1, 3, 0, nullptr};
+    TIMED("basis", (launch_basis<TC, TX>(ctx->dsys, pts, (TX *)ctx->X, ctx->kpad, ctx->ldp,
+                                        ctx->stream)));
+    const int n = ctx->n[sp];
+    GemmOut go{ctx->vrow, (long)ctx->npad[sp] * ctx->ldv, ctx->ldv, 1, 0, 1, nullptr};
+    TIMED("orbital_gemm", (launch_orbital_gemm<TX, TO>((const TX *)ctx->Ct[sp], ctx->npad[sp], n,
+                                                      (const TX *)ctx->X, ctx->kpad, ctx->ldp,
+                                                      (long)num, ctx->kpad, go, ctx->stream)));
+    return QMCB_OK;
+  }
+
+  static MoveArgs move_args(qmcb_ctx *ctx, int m, int mode, const qmcb_results &out) {
+    const int sp = m < ctx->n[0] ? 0 : 1;
+    MoveArgs a{};
+    a.s = slater_args(ctx, sp);
+    a.other = slater_args(ctx, 1 - sp);
+    a.moved = m;
+    a.N = ctx->N;
+    a.vrow = ctx->vrow;
+    a.v_s_comp = (long)ctx->npad[sp] * ctx->ldv;
+    a.v_s_orb = ctx->ldv;
+    a.n_nuc = ctx->nn;
+    a.nuc_xyz = ctx->dsys.nuc_xyz;
+    a.nuc_z = ctx->dsys.nuc_z;
+    a.R = ctx->R;
+    a.propR = ctx->propR;
+    a.U = ctx->U; a.Ux = ctx->Ux; a.Uxx = ctx->Uxx; a.Vee = ctx->Vee; a.Ven = ctx->Ven;
+    a.pU = ctx->pU; a.pUx = ctx->pUx; a.pUxx = ctx->pUxx; a.pVee = ctx->pVee; a.pVen = ctx->pVen;
+    a.v_nn = ctx->dsys.v_nn;
+    a.drift_mod = ctx->opt.drift_mod;
+    a.dt = ctx->opt.dt;
+    a.umrigar_a = ctx->opt.umrigar_a;
+    a.out = out;
+    a.st_inv = ctx->st_inv; a.st_grad = ctx->st_grad; a.st_lap = ctx->st_lap;
+    a.st_logdet = ctx->st_logdet; a.st_sgn = ctx->st_sgn; a.st_singular = ctx->st_singular;
+    a.mode = mode;
+    a.gauss = ctx->gauss;
+    a.unif = ctx->unif;
+    a.acc_el = ctx->acc_el; a.acc_el2 = ctx->acc_el2; a.acc_n = ctx->acc_n;
+    a.acc_prop_el = ctx->acc_prop_el; a.cur_el = ctx->cur_el;
+    return a;
+  }
+
+  static qmcb_status sm(qmcb_ctx *ctx, const MoveArgs &a, const int *d_map, int num) {
+    TIMED("sm_move", (launch_sm_move<TS, TO>(a, d_map, num, ctx->stream)));
+    return QMCB_OK;
+  }
+
+  static qmcb_status commit(qmcb_ctx *ctx, const MoveArgs &a, const int *d_map,
+                            const uint8_t *d_mask, int num) {
+    TIMED("commit", (launch_commit<TS, TO>(a, d_map, d_mask, num, ctx->stream)));
+    return QMCB_OK;
+  }
+
+  static qmcb_status refresh(qmcb_ctx *ctx, const int *d_map, int num) {
+    for (int sp = 0; sp < 2; sp++) {
+      if (ctx->n[sp] == 0) continue;
+      SlaterArgs sa = slater_args(ctx, sp);
+      TIMED("lu_inverse", launch_lu_inverse<TS>(sa, d_map, num, ctx->stream));
+      TIMED("traces_all", launch_traces_all<TS>(sa, d_map, num, ctx->stream));
+    }
+    return QMCB_OK;
+  }
+};
+
+typedef Engine<double, double, double, double> EngineF64;
+typedef Engine<float, float, float, float> EngineF32;
+
+#define DISPATCH(ctx, call)                                  \
+  ((ctx)->opt.precision == QMCB_FP64 ? EngineF64::call : EngineF32::call)
+
+qmcb_status check_ids(qmcb_ctx *ctx, int num, const int32_t *ids) {
+  if (num < 0 || num > ctx->Wc) {
+    ctx->err = "num out of range";
+    return QMCB_INVALID_ARG;
+  }
+  for (int i = 0; i < num; i++)
+    if (ids[i] < 0 || ids[i] >= ctx->Wc) {
+      ctx->err = "walker id out of range";
+      return QMCB_INVALID_ARG;
+    }
+  return QMCB_OK;
+}
+
+// group consecutive basis functions of one atom with identical primitive lists
+struct Shells {
+  std::vector<int> atom, prim_off, nprim, bf0, ncart;
+};
+
+Shells make_shells(const qmcb_system *s) {
+  Shells sh;
+  int j = 0;
+  while (j < s->n_basis) {
+    const int a = s->bf_atom[j];
+    const int p0 = s->bf_prim_off[j], np = s->bf_prim_off[j + 1] - p0;
+    int k = j + 1;
+    while (k < s->n_basis && s->bf_atom[k] == a && k - j < 6) {
+      const int q0 = s->bf_prim_off[k], nq = s->bf_prim_off[k + 1] - q0;
+      if (nq != np) break;
+      bool same = true;
+      for (int i = 0; i < np; i++)
+        if (s->prim_alpha[q0 + i] != s->prim_alpha[p0 + i] ||
+            s->prim_coef[q0 + i] != s->prim_coef[p0 + i]) {
+          same = false;
+          break;
+        }
+      if (!same) break;
+      const int L0 = s->bf_pow[3 * j] + s->bf_pow[3 * j + 1] + s->bf_pow[3 * j + 2];
+      const int L1 = s->bf_pow[3 * k] + s->bf_pow[3 * k + 1] + s->bf_pow[3 * k + 2];
+      if (L0 != L1) break;
+      k++;
+    }
+    sh.atom.push_back(a);
+    sh.prim_off.push_back(p0);
+    sh.nprim.push_back(np);
+    sh.bf0.push_back(j);
+    sh.ncart.push_back(k - j);
+    j = k;
+  }
+  return sh;
+}
+
+template <typename T>
+hipError_t upload(qmcb_ctx *ctx, const T **dst, const T *src, size_t count) {
+  T *d = nullptr;
+  hipError_t e = dalloc(ctx, &d, count);
+  if (e != hipSuccess) return e;
+  if (count) e = hipMemcpy(d, src, count * sizeof(T), hipMemcpyHostToDevice);
+  *dst = d;
+  return e;
+}
+
+qmcb_status create_impl(const qmcb_system *s, const qmcb_options *o, qmcb_ctx *ctx) {
+  ctx->opt = *o;
+  if (ctx->opt.dt <= 0) ctx->opt.dt = 0.1;
+  if (ctx->opt.umrigar_a <= 0) ctx->opt.umrigar_a = 0.5;
+  ctx->N = s->n_up + s->n_dn;
+  ctx->n[0] = s->n_up;
+  ctx->n[1] = s->n_dn;
+  ctx->start[0] = 0;
+  ctx->start[1] = s->n_up;
+  ctx->nn = s->n_nuc;
+  ctx->B = s->n_basis;
+  ctx->nprim = s->n_prim;
+  ctx->kpad = ((s->n_basis + 3) / 4) * 4;
+  ctx->Wc = o->max_walkers;
+  const bool f64 = o->precision == QMCB_FP64;
+  ctx->ts = ctx->tx = ctx->to = f64 ? 8 : 4;
+  QCHK(hipSetDevice(o->device));
+  ctx->device = o->device;
+  QCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+
+  // ---- system tables
+  DevSystem &d = ctx->dsys;
+  d.N = ctx->N;
+  d.n[0] = ctx->n[0]; d.n[1] = ctx->n[1];
+  d.start[0] = 0; d.start[1] = ctx->n[0];
+  d.nn = ctx->nn;
+  d.B = ctx->B;
+  d.nprim = ctx->nprim;
+  Shells sh = make_shells(s);
+  ctx->nshell = d.nshell = (int)sh.atom.size();
+  QCHK(upload(ctx, &d.nuc_xyz, s->nuc_xyz, 3 * (size_t)ctx->nn));
+  QCHK(upload(ctx, &d.nuc_z, s->nuc_z, (size_t)ctx->nn));
+  QCHK(upload(ctx, &d.sh_atom, sh.atom.data(), sh.atom.size()));
+  QCHK(upload(ctx, &d.sh_prim_off, sh.prim_off.data(), sh.prim_off.size()));
+  QCHK(upload(ctx, &d.sh_nprim, sh.nprim.data(), sh.nprim.size()));
+  QCHK(upload(ctx, &d.sh_bf0, sh.bf0.data(), sh.bf0.size()));
+  QCHK(upload(ctx, &d.sh_ncart, sh.ncart.data(), sh.ncart.size()));
+  QCHK(upload(ctx, &d.bf_pow, (const int *)s->bf_pow, 3 * (size_t)ctx->B));
+  QCHK(upload(ctx, &d.prim_alpha, s->prim_alpha, (size_t)ctx->nprim));
+  QCHK(upload(ctx, &d.prim_coef, s->prim_coef, (size_t)ctx->nprim));
+  d.ee[0] = s->ee_updn;
+  d.ee[1] = s->ee_upup;
+  d.ee[2] = s->ee_dndn;
+  QCHK(upload(ctx, &d.en[0], s->en_up, (size_t)ctx->nn));
+  QCHK(upload(ctx, &d.en[1], s->en_dn, (size_t)ctx->nn));
+  double vnn = 0.0;
+  for (int i = 0; i < ctx->nn; i++)
+    for (int j = i + 1; j < ctx->nn; j++) {
+      double dx = s->nuc_xyz[3 * i] - s->nuc_xyz[3 * j];
+      double dy = s->nuc_xyz[3 * i + 1] - s->nuc_xyz[3 * j + 1];
+      double dz = s->nuc_xyz[3 * i + 2] - s->nuc_xyz[3 * j + 2];
+      vnn += s->nuc_z[i] * s->nuc_z[j] / std::sqrt(dx * dx + dy * dy + dz * dz);
+    }
+  d.v_nn = vnn;
+
+  // ---- transposed, padded MO coefficients in the operand type
+  for (int sp = 0; sp < 2; sp++) {
+    const int n = ctx->n[sp];
+    ctx->npad[sp] = pad_orb(std::max(n, 1));
+    const double *C = sp == 0 ? s->mo_up : s->mo_dn;
+    const size_t cnt = (size_t)ctx->kpad * ctx->npad[sp];
+    std::vector<double> ct(cnt, 0.0);
+    for (int o2 = 0; o2 < n; o2++)
+      for (int b = 0; b < ctx->B; b++) ct[(size_t)b * ctx->npad[sp] + o2] = C[(size_t)o2 * ctx->B + b];
+    QCHK(dalloc_bytes(ctx, &ctx->Ct[sp], cnt * ctx->tx));
+    if (f64) {
+      QCHK(hipMemcpy(ctx->Ct[sp], ct.data(), cnt * 8, hipMemcpyHostToDevice));
+    } else {
+      std::vector<float> cf(cnt);
+      for (size_t i = 0; i < cnt; i++) cf[i] = (float)ct[i];
+      QCHK(hipMemcpy(ctx->Ct[sp], cf.data(), cnt * 4, hipMemcpyHostToDevice));
+    }
+  }
+
+  // ---- resident walker state
+  const long W = ctx->Wc, N = ctx->N;
+  QCHK(dalloc(ctx, &ctx->R, W * N * 3));
+  for (int sp = 0; sp < 2; sp++) {
+    const long n = ctx->n[sp];
+    QCHK(dalloc(ctx, &ctx->inv[sp], W * n * n));
+    QCHK(dalloc_bytes(ctx, &ctx->dmat[sp], (size_t)W * kComp * n * n * ctx->ts));
+    QCHK(dalloc(ctx, &ctx->logdet[sp], W));
+    QCHK(dalloc(ctx, &ctx->sgn[sp], W));
+    QCHK(dalloc(ctx, &ctx->gradPR[sp], W * std::max(n, 1L) * 3));
+    QCHK(dalloc(ctx, &ctx->lapPR[sp], W));
+    QCHK(dalloc(ctx, &ctx->singular[sp], W));
+  }
+  QCHK(dalloc(ctx, &ctx->U, W));
+  QCHK(dalloc(ctx, &ctx->Ux, W * N * 3));
+  QCHK(dalloc(ctx, &ctx->Uxx, W));
+  QCHK(dalloc(ctx, &ctx->Vee, W));
+  QCHK(dalloc(ctx, &ctx->Ven, W));
+  QCHK(dalloc(ctx, &ctx->cur_el, W));
+  QCHK(dalloc(ctx, &ctx->Rin, W * N * 3));
+  QCHK(dalloc(ctx, &ctx->propR, W * 3));
+  QCHK(dalloc(ctx, &ctx->gauss, W * 3));
+  QCHK(dalloc(ctx, &ctx->unif, W));
+  QCHK(dalloc(ctx, &ctx->pU, W));
+  QCHK(dalloc(ctx, &ctx->pUx, W * N * 3));
+  QCHK(dalloc(ctx, &ctx->pUxx, W));
+  QCHK(dalloc(ctx, &ctx->pVee, W));
+  QCHK(dalloc(ctx, &ctx->pVen, W));
+  ctx->ldv = ((W + 63) / 64) * 64;
+  const int npmax = std::max(ctx->npad[0], ctx->npad[1]);
+  QCHK(dalloc_bytes(ctx, &ctx->vrow, (size_t)kComp * npmax * ctx->ldv * ctx->to));
+  QCHK(dalloc(ctx, &ctx->acc_el, W));
+  QCHK(dalloc(ctx, &ctx->acc_el2, W));
+  QCHK(dalloc(ctx, &ctx->acc_n, W));
+  QCHK(dalloc(ctx, &ctx->acc_prop_el, W));
+  QCHK(dalloc(ctx, &ctx->d_ids, W));
+  QCHK(dalloc(ctx, &ctx->d_mask, W));
+  // basis scratch: at least one point per walker; all-electron chunks reuse it
+  const size_t per_point = (size_t)kComp * ctx->kpad * ctx->tx;
+  const size_t budget = (size_t)1 << 31;  // 2 GiB of basis values per chunk
+  long cap = std::max<long>(W, std::min<long>(W * std::max(ctx->n[0], 1), (long)(budget / per_point)));
+  cap = std::max<long>(cap, std::max(ctx->n[0], 1));
+  ctx->ldp = ((cap + 63) / 64) * 64;
+  QCHK(dalloc_bytes(ctx, &ctx->X, per_point * ctx->ldp));
+  // device outputs (batch rows)
+  qmcb_results &r = ctx->dout;
+  QCHK(dalloc(ctx, &r.log_psi, W));
+  QCHK(dalloc(ctx, &r.sign_psi, W));
+  QCHK(dalloc(ctx, &r.grad_lnpsi, W * N * 3));
+  QCHK(dalloc(ctx, &r.mod_drift, W * N * 3));
+  QCHK(dalloc(ctx, &r.lap_ratio, W));
+  QCHK(dalloc(ctx, &r.e_local, W));
+  QCHK(dalloc(ctx, &r.kinetic, W));
+  QCHK(dalloc(ctx, &r.potential, W));
+  QCHK(dalloc(ctx, &r.v_ne, W));
+  QCHK(dalloc(ctx, &r.v_ee, W));
+  QCHK(dalloc(ctx, &r.det_ratio, W));
+  QCHK(dalloc(ctx, &r.jastrow_u, W));
+  QCHK(dalloc(ctx, &r.singular, W));
+  ctx->initialized.assign(ctx->Wc, 0);
+  return QMCB_OK;
+}
+
+qmcb_status ensure_staging(qmcb_ctx *ctx) {
+  if (ctx->st_inv) return QMCB_OK;
+  const long W = ctx->Wc;
+  const long nm = std::max(ctx->n[0], ctx->n[1]);
+  QCHK(dalloc(ctx, &ctx->st_inv, W * nm * nm));
+  QCHK(dalloc(ctx, &ctx->st_grad, W * nm * 3));
+  QCHK(dalloc(ctx, &ctx->st_lap, W));
+  QCHK(dalloc(ctx, &ctx->st_logdet, W));
+  QCHK(dalloc(ctx, &ctx->st_sgn, W));
+  QCHK(dalloc(ctx, &ctx->st_singular, W));
+  return QMCB_OK;
+}
+
+template <typename T>
+hipError_t d2h(qmcb_ctx *ctx, T *host, const T *dev, size_t count) {
+  if (!host || !dev || count == 0) return hipSuccess;
+  return hipMemcpyAsync(host, dev, count * sizeof(T), hipMemcpyDeviceToHost, ctx->stream);
+}
+
+qmcb_status copy_out(qmcb_ctx *ctx, int num, qmcb_results *out) {
+  if (!out) return QMCB_OK;
+  const size_t N3 = (size_t)ctx->N * 3;
+  const qmcb_results &d = ctx->dout;
+  QCHK(d2h(ctx, out->log_psi, d.log_psi, num));
+  QCHK(d2h(ctx, out->sign_psi, d.sign_psi, num));
+  QCHK(d2h(ctx, out->grad_lnpsi, d.grad_lnpsi, num * N3));
+  QCHK(d2h(ctx, out->mod_drift, d.mod_drift, num * N3));
+  QCHK(d2h(ctx, out->lap_ratio, d.lap_ratio, num));
+  QCHK(d2h(ctx, out->e_local, d.e_local, num));
+  QCHK(d2h(ctx, out->kinetic, d.kinetic, num));
+  QCHK(d2h(ctx, out->potential, d.potential, num));
+  QCHK(d2h(ctx, out->v_ne, d.v_ne, num));
+  QCHK(d2h(ctx, out->v_ee, d.v_ee, num));
+  QCHK(d2h(ctx, out->det_ratio, d.det_ratio, num));
+  QCHK(d2h(ctx, out->jastrow_u, d.jastrow_u, num));
+  QCHK(d2h(ctx, out->singular, d.singular, num));
+  QCHK(hipStreamSynchronize(ctx->stream));
+  return QMCB_OK;
+}
+
+// evaluate with R already on the device (batch rows) and ids on the device
+qmcb_status evaluate_core(qmcb_ctx *ctx, int num, const int *d_ids, const double *d_R,
+                          int whichE, const qmcb_results &dout) {
+  if (whichE == -1) {
+    QCHK(launch_scatter_rows(ctx->R, d_R, d_ids, num, ctx->N * 3, ctx->stream));
+    qmcb_status s = DISPATCH(ctx, all_electron(ctx, d_ids, num));
+    if (s != QMCB_OK) return s;
+    return DISPATCH(ctx, assemble(ctx, d_ids, num, dout));
+  }
+  qmcb_status s = ensure_staging(ctx);
+  if (s != QMCB_OK) return s;
+  s = DISPATCH(ctx, move_front(ctx, d_ids, num, whichE));
+  if (s != QMCB_OK) return s;
+  MoveArgs a = DISPATCH(ctx, move_args(ctx, whichE, 0, dout));
+  return DISPATCH(ctx, sm(ctx, a, d_ids, num));
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t qmcb_abi_version(void) { return QMCB_ABI_VERSION; }
+
+const char *qmcb_last_error(const qmcb_ctx *ctx) {
+  return ctx ? ctx->err.c_str() : g_create_error.c_str();
+}
+
+qmcb_status qmcb_create(const qmcb_system *sys, const qmcb_options *opt, qmcb_ctx **out) {
+  g_create_error.clear();
+  if (!sys || !opt || !out) {
+    g_create_error = "null argument";
+    return QMCB_INVALID_ARG;
+  }
+  *out = nullptr;
+  if (sys->n_up < 0 || sys->n_dn < 0 || sys->n_dn > sys->n_up || sys->n_up + sys->n_dn < 1 ||
+      sys->n_up > 128 || sys->n_nuc < 1 || sys->n_basis < 1 || sys->n_prim < 1 ||
+      opt->max_walkers < 1 || sys->n_up + sys->n_dn > 512) {
+    g_create_error = "system sizes out of range (1 <= n_dn <= n_up <= 128 electrons per spin)";
+    return QMCB_INVALID_ARG;
+  }
+  if (opt->precision != QMCB_FP64 && opt->precision != QMCB_MIXED_FP32) {
+    g_create_error = "precision not supported by this build";
+    return QMCB_INVALID_ARG;
+  }
+  for (int j = 0; j < sys->n_basis; j++) {
+    const int *p = sys->bf_pow + 3 * j;
+    if (p[0] < 0 || p[1] < 0 || p[2] < 0 || p[0] > 2 || p[1] > 2 || p[2] > 2 ||
+        p[0] + p[1] + p[2] > 2 || sys->bf_atom[j] < 0 || sys->bf_atom[j] >= sys->n_nuc) {
+      g_create_error = "basis function powers must be s/p/d (L <= 2)";
+      return QMCB_INVALID_ARG;
+    }
+  }
+  qmcb_ctx *ctx = new (std::nothrow) qmcb_ctx();
+  if (!ctx) return QMCB_OOM;
+  qmcb_status st = create_impl(sys, opt, ctx);
+  if (st != QMCB_OK) {
+    g_create_error = ctx->err;
+    qmcb_destroy(ctx);
+    return st;
+  }
+  *out = ctx;
+  return QMCB_OK;
+}
+
+void qmcb_destroy(qmcb_ctx *ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (void *p : ctx->allocs) (void)hipFree(p);
+  for (hipEvent_t e : ctx->tm.pool) (void)hipEventDestroy(e);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+qmcb_status qmcb_evaluate(qmcb_ctx *ctx, int32_t num, const int32_t *ids, const double *R,
+                          int32_t whichE, qmcb_results *out) {
+  if (!ctx || (num > 0 && (!ids || !R))) return QMCB_INVALID_ARG;
+  QCHK(hipSetDevice(ctx->device));
+  qmcb_status s = check_ids(ctx, num, ids);
+  if (s != QMCB_OK) return s;
+  if (whichE < -1 || whichE >= ctx->N) {
+    ctx->err = "whichE out of range";
+    return QMCB_INVALID_ARG;
+  }
+  if (num == 0) return QMCB_OK;
+  if (whichE >= 0) {
+    for (int i = 0; i < num; i++)
+      if (!ctx->initialized[ids[i]]) {
+        ctx->err = "single-electron evaluate of a walker without an all-electron evaluate";
+        return QMCB_BAD_STATE;
+      }
+  }
+  QCHK(hipMemcpyAsync(ctx->d_ids, ids, sizeof(int) * num, hipMemcpyHostToDevice, ctx->stream));
+  if (whichE == -1) {
+    QCHK(hipMemcpyAsync(ctx->Rin, R, sizeof(double) * num * ctx->N * 3, hipMemcpyHostToDevice,
+                        ctx->stream));
+  } else {
+    std::vector<double> pr((size_t)num * 3);
+    for (int i = 0; i < num; i++)
+      for (int k = 0; k < 3; k++) pr[3 * i + k] = R[((size_t)i * ctx->N + whichE) * 3 + k];
+    QCHK(hipMemcpyAsync(ctx->propR, pr.data(), sizeof(double) * num * 3, hipMemcpyHostToDevice,
+                        ctx->stream));
+  }
+  s = evaluate_core(ctx, num, ctx->d_ids, ctx->Rin, whichE, ctx->dout);
+  if (s != QMCB_OK) return s;
+  s = copy_out(ctx, num, out);
+  if (s != QMCB_OK) return s;
+  if (whichE == -1) {
+    for (int i = 0; i < num; i++) ctx->initialized[ids[i]] = 1;
+    ctx->staged_moved = -1;
+  } else {
+    ctx->staged_moved = whichE;
+    ctx->staged_num = num;
+    ctx->staged_ids.assign(ids, ids + num);
+    ctx->staged_row.clear();
+    for (int i = 0; i < num; i++) ctx->staged_row[ids[i]] = i;
+  }
+  return QMCB_OK;
+}
+
+qmcb_status qmcb_evaluate_device(qmcb_ctx *ctx, int32_t num, const int32_t *ids_dev,
+                                 const double *R_dev, int32_t whichE, qmcb_results *out_dev) {
+  if (!ctx || !out_dev || (num > 0 && (!ids_dev || !R_dev))) return QMCB_INVALID_ARG;
+  if (num < 0 || num > ctx->Wc || whichE < -1 || whichE >= ctx->N) return QMCB_INVALID_ARG;
+  QCHK(hipSetDevice(ctx->device));
+  if (whichE >= 0) {
+    // the proposal row of each walker: R_dev[b][whichE]
+    std::vector<double> dummy;
+    (void)dummy;
+    QCHK(hipMemcpy2DAsync(ctx->propR, 3 * sizeof(double), R_dev + 3L * whichE,
+                          (size_t)ctx->N * 3 * sizeof(double), 3 * sizeof(double), num,
+                          hipMemcpyDeviceToDevice, ctx->stream));
+  }
+  return evaluate_core(ctx, num, ids_dev, R_dev, whichE, *out_dev);
+}
+
+qmcb_status qmcb_accept(qmcb_ctx *ctx, int32_t num, const int32_t *ids, const uint8_t *mask) {
+  if (!ctx || (num > 0 && (!ids || !mask))) return QMCB_INVALID_ARG;
+  if (ctx->staged_moved < 0) {
+    ctx->err = "no staged single-electron proposal";
+    return QMCB_BAD_STATE;
+  }
+  QCHK(hipSetDevice(ctx->device));
+  std::vector<uint8_t> rowmask(ctx->staged_num, 0);
+  for (int i = 0; i < num; i++) {
+    auto it = ctx->staged_row.find(ids[i]);
+    if (it == ctx->staged_row.end()) {
+      ctx->err = "walker id was not part of the staged proposal";
+      return QMCB_INVALID_ARG;
+    }
+    rowmask[it->second] = mask[i] ? 1 : 0;
+  }
+  QCHK(hipMemcpyAsync(ctx->d_ids, ctx->staged_ids.data(), sizeof(int) * ctx->staged_num,
+                      hipMemcpyHostToDevice, ctx->stream));
+  QCHK(hipMemcpyAsync(ctx->d_mask, rowmask.data(), ctx->staged_num, hipMemcpyHostToDevice,
+                      ctx->stream));
+  MoveArgs a = DISPATCH(ctx, move_args(ctx, ctx->staged_moved, 0, ctx->dout));
+  qmcb_status s = DISPATCH(ctx, commit(ctx, a, ctx->d_ids, ctx->d_mask, ctx->staged_num));
+  if (s != QMCB_OK) return s;
+  QCHK(hipStreamSynchronize(ctx->stream));
+  ctx->staged_moved = -1;
+  return QMCB_OK;
+}
+
+qmcb_status qmcb_refresh(qmcb_ctx *ctx, int32_t num, const int32_t *ids) {
+  if (!ctx) return QMCB_INVALID_ARG;
+  QCHK(hipSetDevice(ctx->device));
+  qmcb_status s = check_ids(ctx, num, ids);
+  if (s != QMCB_OK) return s;
+  QCHK(hipMemcpyAsync(ctx->d_ids, ids, sizeof(int) * num, hipMemcpyHostToDevice, ctx->stream));
+  s = DISPATCH(ctx, refresh(ctx, ctx->d_ids, num));
+  if (s != QMCB_OK) return s;
+  QCHK(hipStreamSynchronize(ctx->stream));
+  return QMCB_OK;
+}
+
+qmcb_status qmcb_get_positions(qmcb_ctx *ctx, int32_t num, double *R) {
+  if (!ctx || num < 0 || num > ctx->Wc || (num > 0 && !R)) return QMCB_INVALID_ARG;
+  QCHK(hipSetDevice(ctx->device));
+  QCHK(hipMemcpyAsync(R, ctx->R, sizeof(double) * num * ctx->N * 3, hipMemcpyDeviceToHost,
+                      ctx->stream));
+  QCHK(hipStreamSynchronize(ctx->stream));
+  return QMCB_OK;
+}
+
+qmcb_status qmcb_vmc_steps(qmcb_ctx *ctx, int32_t num, int64_t first_step, int32_t n_steps,
+                           double tau, qmcb_vmc_stats *stats) {
+  if (!ctx || num < 1 || num > ctx->Wc || n_steps < 0 || !(tau > 0)) return QMCB_INVALID_ARG;
+  if (ctx->opt.drift_mod != QMCB_DRIFT_NONE) {
+    ctx->err = "device VMC supports QF_modification_type none";
+    return QMCB_INVALID_ARG;
+  }
+  for (int i = 0; i < num; i++)
+    if (!ctx->initialized[i]) {
+      ctx->err = "vmc slots must be initialised by an all-electron evaluate";
+      return QMCB_BAD_STATE;
+    }
+  QCHK(hipSetDevice(ctx->device));
+  QCHK(hipMemsetAsync(ctx->acc_el, 0, sizeof(double) * num, ctx->stream));
+  QCHK(hipMemsetAsync(ctx->acc_el2, 0, sizeof(double) * num, ctx->stream));
+  QCHK(hipMemsetAsync(ctx->acc_n, 0, sizeof(double) * num, ctx->stream));
+  QCHK(hipMemsetAsync(ctx->acc_prop_el, 0, sizeof(double) * num, ctx->stream));
+  qmcb_results none{};
+  for (int s = 0; s < n_steps; s++) {
+    const long step = first_step + s;
+    const int m = (int)(step % ctx->N);
+    ProposeArgs pa{};
+    pa.N = ctx->N;
+    pa.moved = m;
+    pa.n_up = ctx->n[0];
+    pa.R = ctx->R;
+    pa.gradPR[0] = ctx->gradPR[0];
+    pa.gradPR[1] = ctx->gradPR[1];
+    pa.Ux = ctx->Ux;
+    pa.tau = tau;
+    pa.seed = ctx->opt.seed;
+    pa.step = step;
+    pa.propR = ctx->propR;
+    pa.gauss = ctx->gauss;
+    pa.unif = ctx->unif;
+    TIMED("propose", launch_propose(pa, nullptr, num, ctx->stream));
+    qmcb_status st = DISPATCH(ctx, move_front(ctx, nullptr, num, m));
+    if (st != QMCB_OK) return st;
+    MoveArgs a = DISPATCH(ctx, move_args(ctx, m, 1, none));
+    a.tau = tau;
+    st = DISPATCH(ctx, sm(ctx, a, nullptr, num));
+    if (st != QMCB_OK) return st;
+    if (ctx->opt.refresh_interval > 0 && (step + 1) % ctx->opt.refresh_interval == 0) {
+      st = DISPATCH(ctx, refresh(ctx, nullptr, num));
+      if (st != QMCB_OK) return st;
+    }
+  }
+  if (stats) {
+    std::vector<double> a(num), b(num), c(num), d(num);
+    QCHK(hipMemcpyAsync(a.data(), ctx->acc_el, sizeof(double) * num, hipMemcpyDeviceToHost, ctx->stream));
+    QCHK(hipMemcpyAsync(b.data(), ctx->acc_el2, sizeof(double) * num, hipMemcpyDeviceToHost, ctx->stream));
+    QCHK(hipMemcpyAsync(c.data(), ctx->acc_n, sizeof(double) * num, hipMemcpyDeviceToHost, ctx->stream));
+    QCHK(hipMemcpyAsync(d.data(), ctx->acc_prop_el, sizeof(double) * num, hipMemcpyDeviceToHost, ctx->stream));
+    QCHK(hipStreamSynchronize(ctx->stream));
+    stats->sum_e_local = stats->sum_e_local_sq = stats->sum_accept = stats->sum_prop_e_local = 0;
+    for (int i = 0; i < num; i++) {
+      stats->sum_e_local += a[i];
+      stats->sum_e_local_sq += b[i];
+      stats->sum_accept += c[i];
+      stats->sum_prop_e_local += d[i];
+    }
+    stats->n_samples = (int64_t)num * n_steps;
+  }
+  tm_collect(ctx);
+  return QMCB_OK;
+}
+
+qmcb_status qmcb_sync(qmcb_ctx *ctx) {
+  if (!ctx) return QMCB_INVALID_ARG;
+  QCHK(hipSetDevice(ctx->device));
+  QCHK(hipStreamSynchronize(ctx->stream));
+  tm_collect(ctx);
+  return QMCB_OK;
+}
+
+void qmcb_enable_kernel_timing(qmcb_ctx *ctx, int32_t on) {
+  if (!ctx) return;
+  ctx->timing = on != 0;
+  ctx->tm.names.clear();
+  ctx->tm.index.clear();
+  ctx->tm.ms.clear();
+  ctx->tm.launches.clear();
+  ctx->tm.marks.clear();
+  ctx->tm.used = 0;
+}
+
+int32_t qmcb_kernel_times(qmcb_ctx *ctx, int32_t cap, const char **names, double *ms,
+                          int64_t *launches) {
+  if (!ctx) return 0;
+  tm_collect(ctx);
+  const int n = (int)ctx->tm.names.size();
+  for (int i = 0; i < n && i < cap; i++) {
+    if (names) names[i] = ctx->tm.names[i].c_str();
+    if (ms) ms[i] = ctx->tm.ms[i];
+    if (launches) launches[i] = ctx->tm.launches[i];
+  }
+  return n;
+}
+
+}  // extern "C"

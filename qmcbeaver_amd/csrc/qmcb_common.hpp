@@ -1,0 +1,101 @@
+// qmcb_common.hpp -- device-side data structures shared by the kernels.
+//
+// Layout of the resident walker state in HBM (one context, capacity Wc):
+//   R        [Wc][N][3]           FP64 positions, alpha electrons first
+//   inv[s]   [Wc][n][n]           FP64 inverse Slater matrix, element (o,e)
+//   dmat[s]  [Wc][5][n][n]        Tstore, component k in {value, d/dx, d/dy,
+//                                 d/dz, Laplacian}, stored transposed:
+//                                 dmat(k, o, e) = d_k D(e, o)
+//   logdet[s], sgn[s] [Wc]        FP64 ln|det|, sign (+1/-1)
+//   gradPR[s] [Wc][n][3]          FP64 grad det / det per electron
+//   lapPR[s]  [Wc]                FP64 Laplacian det / det
+//   U, Uxx, Vee, Ven [Wc]; Ux [Wc][N][3]   FP64 Jastrow / potential sums
+// The transposed derivative storage lets one thread own one electron column
+// of A^-1 and stream the matching column of every derivative matrix with
+// coalesced loads (64 lanes = 64 consecutive electrons).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/qmcbeaver_amd.h"
+
+namespace qmcb {
+
+constexpr int kComp = 5;  // value, x, y, z, Laplacian
+
+struct DevSystem {
+  int N, n[2], start[2], nn, B, nshell, nprim;
+  const double *nuc_xyz, *nuc_z;
+  const int *sh_atom, *sh_prim_off, *sh_nprim, *sh_bf0, *sh_ncart;
+  const int *bf_pow;  // [B][3]
+  const double *prim_alpha, *prim_coef;
+  qmcb_pade ee[3];                    // up-dn, up-up, dn-dn
+  const qmcb_pade *en[2];             // [nn] per spin
+  double v_nn;
+};
+
+// ---- Pade u(r) = P(r)/Q(r), P(0)=0, Q(0)=1 (PadeCorrelationFunction.cpp:64-83)
+__device__ __forceinline__ void pade_eval(const qmcb_pade &f, double r, double &u,
+                                          double &du, double &d2u) {
+  // Horner for value, first and second derivative (Polynomial.cpp:127-149)
+  double a = 0.0, ap = 0.0, app = 0.0;
+  if (f.n_num > 0) {
+    a = f.num[f.n_num - 1];
+    for (int i = f.n_num - 2; i >= -1; i--) {
+      double c = i >= 0 ? f.num[i] : 0.0;
+      app = app * r + ap;
+      ap = ap * r + a;
+      a = a * r + c;
+    }
+    app *= 2.0;
+  }
+  double b = 1.0, bp = 0.0, bpp = 0.0;
+  if (f.n_den > 0) {
+    b = f.den[f.n_den - 1];
+    for (int i = f.n_den - 2; i >= -1; i--) {
+      double c = i >= 0 ? f.den[i] : 1.0;
+      bpp = bpp * r + bp;
+      bp = bp * r + b;
+      b = b * r + c;
+    }
+    bpp *= 2.0;
+  }
+  u = a / b;
+  du = ap / b - a * bp / b / b;
+  d2u = app / b - 2 * ap * bp / b / b + 2 * a * bp * bp / b / b / b - a * bpp / b / b;
+}
+
+// ---- wavefront reductions (64 lanes) ---------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// ---- Philox4x32-10 (Salmon et al., SC'11) counter-based RNG ---------------
+struct u4 { uint32_t x, y, z, w; };
+
+__host__ __device__ inline u4 philox4x32(u4 ctr, uint32_t k0, uint32_t k1) {
+  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+  const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+  for (int i = 0; i < 10; i++) {
+    uint64_t p0 = (uint64_t)M0 * ctr.x, p1 = (uint64_t)M1 * ctr.z;
+    uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    u4 n = {hi1 ^ ctr.y ^ k0, lo1, hi0 ^ ctr.w ^ k1, lo0};
+    ctr = n;
+    k0 += W0;
+    k1 += W1;
+  }
+  return ctr;
+}
+
+// uniform in (0,1] from 32 random bits pair (53-bit mantissa)
+__host__ __device__ inline double u01(uint32_t a, uint32_t b) {
+  uint64_t v = ((uint64_t)a << 21) ^ (uint64_t)b;
+  v &= ((1ULL << 53) - 1);
+  return ((double)v + 1.0) * (1.0 / 9007199254740992.0);
+}
+
+}  // namespace qmcb

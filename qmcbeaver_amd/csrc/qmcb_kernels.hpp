@@ -1,0 +1,147 @@
+// qmcb_kernels.hpp -- kernel launch interfaces (host side) and the POD
+// argument records shared by the kernels.
+#pragma once
+
+#include <hip/hip_bf16.h>
+#include <hip/hip_runtime.h>
+
+#include "qmcb_common.hpp"
+
+namespace qmcb {
+
+// A batch of evaluation points: point p lives at
+//   base + (p / per_walker) * walker_stride + (p % per_walker) * point_stride
+//   (with walker_map: base + walker_map[p / per_walker] * walker_stride + ...)
+struct PointSet {
+  const double *base;
+  long P;
+  int per_walker;
+  long walker_stride;
+  int point_stride;
+  const int *walker_map;
+};
+
+// Output addressing of the orbital GEMM: element (component k, orbital o,
+// point p) is stored at
+//   out + k*s_comp + o*s_orb + (p / per_walker)*s_walker + (p % per_walker)*s_elec
+//   (the walker index goes through walker_map when it is non-null)
+struct GemmOut {
+  void *out;
+  long s_comp, s_orb, s_walker, s_elec;
+  int per_walker;
+  const int *walker_map;
+};
+
+// scatter batch rows of positions into resident slots: R[map[b]] = Rin[b]
+hipError_t launch_scatter_rows(double *R, const double *Rin, const int *map, int num, int row_len,
+                               hipStream_t st);
+
+// ---- kernel (a) ----
+template <typename TC, typename TS>
+hipError_t launch_basis(const DevSystem &sys, const PointSet &pts, TS *X, int ldk, long ldp,
+                        hipStream_t st);
+
+// ---- kernel (b): D^T[k][o][p] = sum_b Ct[b][o] X[k][b][p] on MFMA ----
+// TI: operand type (double, float, bf16); TO: output type (double/float)
+template <typename TI, typename TO>
+hipError_t launch_orbital_gemm(const TI *Ct, int ldc, int n_orb, const TI *X, int ldk, long ldp,
+                               long P, int kpad, const GemmOut &o, hipStream_t st);
+
+// ---- kernel (c): Slater inverse, SM updates, traces ----
+struct SlaterArgs {
+  int n;                 // electrons of this spin
+  int N;                 // all electrons
+  int spin_start;        // first electron index of this spin
+  double *inv;           // [Wc][n][n]
+  void *dmat;            // [Wc][5][n][n] Tstore
+  double *logdet, *sgn;  // [Wc]
+  double *gradPR;        // [Wc][n][3]
+  double *lapPR;         // [Wc]
+  int *singular;         // [Wc]
+};
+
+template <typename TS>
+hipError_t launch_lu_inverse(const SlaterArgs &a, const int *walkers, int num, hipStream_t st);
+template <typename TS>
+hipError_t launch_traces_all(const SlaterArgs &a, const int *walkers, int num, hipStream_t st);
+
+// Single-electron proposal: everything the evaluate contract needs at the
+// proposed position, plus optional in-kernel Metropolis and commit.
+struct MoveArgs {
+  SlaterArgs s;          // moved spin
+  SlaterArgs other;      // the other spin (read only)
+  int moved;             // moved electron (global index)
+  int N;
+  const void *vrow;      // new rows [5][ldo_orb][Wc] (GEMM output type TO)
+  long v_s_comp, v_s_orb;
+  int n_nuc;
+  const double *nuc_xyz, *nuc_z;
+  double *R;             // [Wc][N][3] current positions
+  const double *propR;   // [Wc][3] proposed position of `moved`
+  double *U, *Ux, *Uxx, *Vee, *Ven;           // current Jastrow/potential
+  const double *pU, *pUx, *pUxx, *pVee, *pVen;  // at the proposal
+  double v_nn;
+  int drift_mod;
+  double dt, umrigar_a;
+  // outputs at the proposal (device, any may be null)
+  qmcb_results out;
+  // staging of the proposal (staged mode)
+  double *st_inv, *st_grad, *st_lap, *st_logdet, *st_sgn;
+  int *st_singular;
+  // fused Metropolis (vmc mode)
+  int mode;              // 0 = stage, 1 = vmc accept in kernel
+  double tau;
+  const double *gauss;   // [Wc][3] the Gaussian displacement used (vmc)
+  const double *unif;    // [Wc] uniform for the accept test (vmc)
+  double *acc_el, *acc_el2, *acc_n, *acc_prop_el, *cur_el;  // [Wc]
+  void *dmat_refresh_dummy;
+};
+
+template <typename TS, typename TO>
+hipError_t launch_sm_move(const MoveArgs &a, const int *walkers, int num, hipStream_t st);
+
+// ---- kernel (d): Jastrow + Coulomb, all-electron and single-electron ----
+struct JastrowArgs {
+  DevSystem sys;
+  const double *R;       // [Wc][N][3]
+  double *U, *Ux, *Uxx, *Vee, *Ven;   // outputs
+  // single-electron
+  int moved;
+  const double *propR;   // [Wc][3]
+};
+hipError_t launch_jastrow_all(const JastrowArgs &a, const int *walkers, int num, hipStream_t st);
+hipError_t launch_jastrow_move(const JastrowArgs &a, const double *U, const double *Ux,
+                               const double *Uxx, const double *Vee, const double *Ven,
+                               const int *walkers, int num, hipStream_t st);
+
+// all-electron assembly: psi, gradient, Laplacian, E_L, drift
+struct AssembleArgs {
+  DevSystem sys;
+  SlaterArgs s[2];
+  const double *R, *U, *Ux, *Uxx, *Vee, *Ven;
+  int drift_mod;
+  double dt, umrigar_a;
+  qmcb_results out;      // device arrays indexed by batch row
+  double *cur_el;        // [Wc] current E_L (resident)
+};
+hipError_t launch_assemble(const AssembleArgs &a, const int *walkers, int num, hipStream_t st);
+
+// proposal of a single-electron importance-sampled move (QMCWalker.cpp:343-398)
+struct ProposeArgs {
+  int N, moved, n_up;
+  const double *R;
+  const double *gradPR[2];
+  const double *Ux;
+  double tau;
+  uint64_t seed;
+  long step;
+  double *propR, *gauss, *unif;
+};
+hipError_t launch_propose(const ProposeArgs &a, const int *walkers, int num, hipStream_t st);
+
+// commit staged single-electron proposals
+template <typename TS, typename TO>
+hipError_t launch_commit(const MoveArgs &a, const int *walkers, const uint8_t *mask, int num,
+                         hipStream_t st);
+
+}  // namespace qmcb

@@ -1,0 +1,62 @@
+"""C-ABI library checks that need no GPU: it loads, exports every function
+declared in include/qmcbeaver_amd.h, and rejects bad arguments before any
+device work."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import qmcbeaver_amd as q
+from qmcbeaver_amd import synth
+from qmcbeaver_amd._abi import Options, SystemBuffers
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    src = open(os.path.join(ROOT, "include", "qmcbeaver_amd.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(qmcb_[a-z_]+)\s*\(", src)))
+
+
+def test_header_declares_expected_api():
+    fns = declared_functions()
+    assert set(q.EXPORTS) == set(fns), (set(q.EXPORTS) ^ set(fns))
+
+
+def test_library_exports_every_declared_symbol():
+    L = q.lib()
+    for f in declared_functions():
+        assert hasattr(L, f), f
+    assert L.qmcb_abi_version() == 1
+
+
+def test_create_rejects_bad_arguments():
+    L = q.lib()
+    s = synth.make_system(0)
+    buf = SystemBuffers(s.pack())
+    o = Options()
+    o.max_walkers = 0
+    h = C.c_void_p()
+    assert L.qmcb_create(C.byref(buf.struct), C.byref(o), C.byref(h)) == 1
+    assert b"out of range" in L.qmcb_last_error(None)
+    o.max_walkers = 4
+    o.precision = 7
+    assert L.qmcb_create(C.byref(buf.struct), C.byref(o), C.byref(h)) == 1
+    assert L.qmcb_create(None, C.byref(o), C.byref(h)) == 1
+    # f functions (L = 3) are outside the s/p/d kernels
+    p = s.pack()
+    p["bf_pow"] = p["bf_pow"].copy()
+    p["bf_pow"][0] = [3, 0, 0]
+    buf2 = SystemBuffers(p)
+    o.precision = 0
+    assert L.qmcb_create(C.byref(buf2.struct), C.byref(o), C.byref(h)) == 1
+
+
+def test_null_context_calls_are_rejected():
+    L = q.lib()
+    assert L.qmcb_evaluate(None, 0, None, None, -1, None) == 1
+    assert L.qmcb_accept(None, 0, None, None) == 1
+    assert L.qmcb_sync(None) == 1

@@ -1,0 +1,155 @@
+"""Parity of the HIP path (through the C ABI) with the reference evaluate:
+the golden vectors written by the reference build and the C oracle.
+
+Bars: FP64 path <= 1e-10 relative (floor 1) for ln|Psi|, grad ln Psi,
+Laplacian(Psi)/Psi, E_L and potentials on identical configurations; sign
+exact.  Single-electron Sherman-Morrison sequences of ill-conditioned random
+Slater matrices: 1e-9 (the reference's own SM drift is ~1e-10 there).
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from qmcbeaver_amd import synth, Context
+from oracle.pyoracle import Oracle
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+CASES = [("cfg0_h2", 0), ("cfg1_organic64", 1), ("cfg2_mol256", 2)]
+
+
+def rel(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    return float(np.max(np.abs(a - b) / np.maximum(1.0, np.abs(b))))
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", params=CASES, ids=[c[0] for c in CASES])
+def golden(request):
+    name, cfg = request.param
+    g = np.load(os.path.join(GOLDEN, name + ".npz"))
+    s = synth.make_system(cfg)
+    assert s.digest() == str(g["digest"])
+    return name, s, g
+
+
+def test_all_electron_fp64_matches_reference(golden):
+    name, s, g = golden
+    R = g["R"]
+    say("case", name, "all-electron fp64, walkers", R.shape[0])
+    with Context(s.pack(), precision="fp64", max_walkers=R.shape[0]) as ctx:
+        res = ctx.evaluate(np.arange(R.shape[0]), R)
+    assert not res["singular"].any()
+    for k in ("log_psi", "e_local", "kinetic", "potential", "v_ne", "v_ee"):
+        assert rel(res[k], g["ev_" + k]) < 1e-10, (k, rel(res[k], g["ev_" + k]))
+    assert rel(res["grad_lnpsi"], g["ev_grad_lnpsi"]) < 1e-10
+    assert rel(res["mod_drift"], g["ev_mod_drift"]) < 1e-10
+    sign_ref = np.sign(g["ev_detA"] * g["ev_detB"])
+    assert np.array_equal(res["sign_psi"], sign_ref.astype(np.int8))
+
+
+def test_all_electron_scattered_ids(golden):
+    """Walker slots given out of order land in the right resident state."""
+    name, s, g = golden
+    R = g["R"][:4]
+    n = R.shape[0]
+    say("case", name, "scattered ids")
+    ids = np.array([3, 1, 0, 2])[:n] + 5
+    with Context(s.pack(), precision="fp64", max_walkers=12) as ctx:
+        res = ctx.evaluate(ids, R)
+        assert rel(res["e_local"], g["ev_e_local"][:n]) < 1e-10
+        pos = ctx.positions(12)
+        np.testing.assert_array_equal(pos[ids], R)
+
+
+def test_umrigar93_drift_matches_reference(golden):
+    name, s, g = golden
+    if "umr_mod_drift" not in g:
+        pytest.skip("no umrigar93 fixture")
+    n = g["umr_mod_drift"].shape[0]
+    say("case", name, "umrigar93 drift")
+    with Context(s.pack(), precision="fp64", max_walkers=n, drift_mod=1, dt=0.1) as ctx:
+        res = ctx.evaluate(np.arange(n), g["R"][:n])
+    assert rel(res["mod_drift"], g["umr_mod_drift"]) < 1e-10
+
+
+def test_single_electron_sequence_matches_reference(golden):
+    name, s, g = golden
+    mv = g["moves"]
+    nw, nm = mv.shape[:2]
+    say("case", name, "single-electron sequence", nw, "walkers", nm, "moves")
+    R = g["R"][:nw].copy()
+    nrec = nm + 1
+    with Context(s.pack(), precision="fp64", max_walkers=nw) as ctx:
+        ids = np.arange(nw)
+        res = ctx.evaluate(ids, R)
+        idx = np.arange(nw) * nrec
+        assert rel(res["e_local"], g["sq_e_local"][idx]) < 1e-10
+        for m in range(nm):
+            e = int(mv[0, m, 0])
+            R[:, e, :] = mv[:, m, 1:]
+            res = ctx.evaluate(ids, R, whichE=e)
+            idx = np.arange(nw) * nrec + m + 1
+            assert not res["singular"].any()
+            assert rel(res["log_psi"], g["sq_log_psi"][idx]) < 1e-9, m
+            assert rel(res["e_local"], g["sq_e_local"][idx]) < 1e-9, m
+            assert rel(res["grad_lnpsi"], g["sq_grad_lnpsi"][idx]) < 1e-9, m
+            ctx.accept(ids, np.ones(nw, dtype=np.uint8))
+
+
+def test_rejected_proposals_leave_state_unchanged():
+    s = synth.make_system(1)
+    R0 = synth.initial_walkers(s, 4, 11)
+    o = Oracle(s.pack())
+    say("case cfg1 reject/accept interleaving")
+    rng = np.random.default_rng(3)
+    with Context(s.pack(), precision="fp64", max_walkers=4) as ctx:
+        ids = np.arange(4)
+        ctx.evaluate(ids, R0)
+        walkers = [o.new_walker() for _ in range(4)]
+        for w in range(4):
+            o.evaluate(walkers[w], R0[w], -1)
+        R = R0.copy()
+        for m in range(24):
+            e = (m * 7) % s.n_elec
+            Rp = R.copy()
+            Rp[:, e, :] += 0.2 * rng.normal(size=(4, 3))
+            res = ctx.evaluate(ids, Rp, whichE=e)
+            mask = rng.random(4) < 0.5
+            for w in range(4):
+                ref = o.evaluate(walkers[w], Rp[w], e)
+                assert rel(res["e_local"][w], ref["e_local"][0]) < 1e-9
+                if not mask[w]:  # reference: re-evaluate at the old position
+                    o.evaluate(walkers[w], R[w], e)
+            ctx.accept(ids, mask.astype(np.uint8))
+            R[mask] = Rp[mask]
+        np.testing.assert_array_equal(ctx.positions(4), R)
+
+
+def test_refresh_equals_fresh_evaluate():
+    s = synth.make_system(1)
+    R = synth.initial_walkers(s, 3, 12)
+    say("case cfg1 refresh")
+    rng = np.random.default_rng(4)
+    with Context(s.pack(), precision="fp64", max_walkers=3) as ctx, \
+            Context(s.pack(), precision="fp64", max_walkers=3) as fresh:
+        ids = np.arange(3)
+        ctx.evaluate(ids, R)
+        for m in range(40):
+            e = m % s.n_elec
+            R[:, e, :] += 0.15 * rng.normal(size=(3, 3))
+            ctx.evaluate(ids, R, whichE=e)
+            ctx.accept(ids, np.ones(3, dtype=np.uint8))
+        ctx.refresh(ids)
+        # a zero move (same position) evaluates the refreshed state
+        res = ctx.evaluate(ids, R, whichE=0)
+        ref = fresh.evaluate(ids, R)
+        assert rel(res["e_local"], ref["e_local"]) < 1e-10
+        assert rel(res["log_psi"], ref["log_psi"]) < 1e-10
+        assert np.allclose(res["det_ratio"], 1.0, rtol=1e-12)
