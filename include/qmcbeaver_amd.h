@@ -88,8 +88,10 @@ typedef struct qmcb_options {
   int32_t precision;        /* qmcb_precision                                  */
   int32_t max_walkers;      /* resident walker slots                           */
   int32_t drift_mod;        /* qmcb_drift_mod                                  */
-  int32_t refresh_interval; /* FP64 re-inversion every K accepted-or-not moves
-                               per walker in single-electron mode; 0 = never   */
+  int32_t refresh_interval; /* device VMC: FP64 re-inversion of a spin's Slater
+                               matrices after K single-electron moves of that
+                               spin; 0 = never (the reference never re-inverts
+                               in single-electron mode)                        */
   double dt;                /* time step used by the drift modification        */
   double umrigar_a;         /* umrigar93_equalelectrons_parameter              */
   uint64_t seed;            /* device RNG seed (vmc entry points)              */

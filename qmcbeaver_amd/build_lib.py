@@ -36,6 +36,14 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stderr)
     os.replace(tmp, LIB)
+    # the C++ drop-in example over include/qmcb_functions.hpp
+    ex = os.path.join(HERE, "examples")
+    r = subprocess.run(["g++", "-std=c++17", "-O2", "-I", os.path.join(os.path.dirname(HERE), "include"),
+                        os.path.join(ex, "dropin_vmc.cpp"), "-L", HERE, "-lqmcb_amd",
+                        "-Wl,-rpath,$ORIGIN/..", "-o", os.path.join(ex, "dropin_vmc")],
+                       capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("drop-in example failed to build:\n" + r.stderr)
     if verbose:
         print("built", LIB)
     return LIB

@@ -18,6 +18,8 @@
 // Mapping: one lane per point (64 points per wavefront), all lanes walk the
 // shell list in lockstep, so shell/primitive parameters are wave-uniform
 // scalar loads and every store X[k][bf][p] is a coalesced 64-wide row.
+// grid.y splits the shell list into cost-balanced chunks so that a batch of
+// a few thousand points (one moved electron per walker) still fills all CUs.
 #include "qmcb_kernels.hpp"
 
 namespace qmcb {
@@ -53,7 +55,8 @@ basis_kernel(DevSystem sys, PointSet pts, TS *__restrict__ X, int ldk, long ldp)
   const long cs = (long)ldk * ldp;
   int prev_atom = -1;
   TC d[3] = {0, 0, 0}, rsq = 0;
-  for (int s = 0; s < sys.nshell; s++) {
+  const int s_beg = sys.chunk_beg[blockIdx.y], s_end = sys.chunk_beg[blockIdx.y + 1];
+  for (int s = s_beg; s < s_end; s++) {
     const int atom = sys.sh_atom[s];
     if (atom != prev_atom) {
       prev_atom = atom;
@@ -125,7 +128,7 @@ template <typename TC, typename TS>
 hipError_t launch_basis(const DevSystem &sys, const PointSet &pts, TS *X, int ldk, long ldp,
                         hipStream_t st) {
   if (pts.P <= 0) return hipSuccess;
-  dim3 grid((unsigned)((pts.P + 255) / 256));
+  dim3 grid((unsigned)((pts.P + 255) / 256), (unsigned)sys.nchunk);
   hipLaunchKernelGGL((basis_kernel<TC, TS>), grid, dim3(256), 0, st, sys, pts, X, ldk, ldp);
   return hipGetLastError();
 }

@@ -52,27 +52,49 @@ orbital_gemm_kernel(const TI *__restrict__ Ct, int ldc, int n_orb, const TI *__r
   typedef typename M::acc_t acc_t;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const long p0 = ((long)blockIdx.x * 4 + wave) * 16;
-  if (p0 >= P) return;  // wave-uniform
+  const bool live = p0 < P;  // wave-uniform
   const int comp = blockIdx.y;
   const int o0 = blockIdx.z * 16 * MT;
   const int li = lane & 15, lk = lane >> 4;
-  const TI *__restrict__ xp = X + (long)comp * ldk * ldp + (long)lk * ldp + p0 + li;
-  const TI *__restrict__ cp = Ct + (long)lk * ldc + o0 + li;
   acc_t acc[MT];
 #pragma unroll
   for (int t = 0; t < MT; t++) acc[t] = acc_t{0, 0, 0, 0};
-
-  for (int k0 = 0; k0 < kpad; k0 += 4) {
-    const TI b = xp[(long)k0 * ldp];
-    TI a[MT];
+  if (live) {
+    const TI *__restrict__ xp = X + (long)comp * ldk * ldp + (long)lk * ldp + p0 + li;
+    const TI *__restrict__ cp = Ct + (long)lk * ldc + o0 + li;
+    for (int k0 = 0; k0 < kpad; k0 += 4) {
+      const TI b = xp[(long)k0 * ldp];
+      TI a[MT];
 #pragma unroll
-    for (int t = 0; t < MT; t++) a[t] = cp[(long)k0 * ldc + 16 * t];
+      for (int t = 0; t < MT; t++) a[t] = cp[(long)k0 * ldc + 16 * t];
 #pragma unroll
-    for (int t = 0; t < MT; t++) acc[t] = M::run(a[t], b, acc[t]);
+      for (int t = 0; t < MT; t++) acc[t] = M::run(a[t], b, acc[t]);
+    }
+  }
+  TO *out = (TO *)o.out;
+  if (o.s_orb == 1 && o.per_walker == 1 && !o.walker_map) {
+    // point-major output (one walker's orbital row contiguous): transpose each
+    // 16x16 tile through LDS so a point's 16 orbitals go out as one segment
+    __shared__ TO tile[4][16][17];
+#pragma unroll
+    for (int t = 0; t < MT; t++) {
+#pragma unroll
+      for (int i = 0; i < 4; i++) tile[wave][li][4 * lk + i] = (TO)acc[t][i];
+      __syncthreads();
+      const int pp = lane >> 2, oo = (lane & 3) * 4;
+      const long p = p0 + pp;
+      if (live && p < P) {
+        TO *dst = out + p * o.s_walker + (long)comp * o.s_comp + o0 + 16 * t + oo;
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          if (o0 + 16 * t + oo + q < n_orb) dst[q] = tile[wave][pp][oo + q];
+      }
+      __syncthreads();
+    }
+    return;
   }
   const long p = p0 + li;
-  if (p >= P) return;
-  TO *out = (TO *)o.out;
+  if (!live || p >= P) return;
   const long row = p / o.per_walker;
   const long wk = o.walker_map ? o.walker_map[row] : row;
   const long base = (long)comp * o.s_comp + wk * o.s_walker + (p % o.per_walker) * o.s_elec;

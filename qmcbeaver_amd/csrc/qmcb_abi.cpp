@@ -53,6 +53,8 @@ struct qmcb_ctx {
   double *logdet[2] = {nullptr, nullptr}, *sgn[2] = {nullptr, nullptr};
   double *gradPR[2] = {nullptr, nullptr}, *lapPR[2] = {nullptr, nullptr};
   int *singular[2] = {nullptr, nullptr};
+  int *dcount[2] = {nullptr, nullptr};
+  double *ubuf[2] = {nullptr, nullptr}, *zbuf[2] = {nullptr, nullptr};
   double *U = nullptr, *Ux = nullptr, *Uxx = nullptr, *Vee = nullptr, *Ven = nullptr;
   double *cur_el = nullptr;
   // batch-row staging
@@ -81,6 +83,7 @@ struct qmcb_ctx {
   std::vector<int> staged_ids;
   std::map<int, int> staged_row;
   std::vector<int> initialized;  // per slot: has a resident state
+  long sm_updates[2] = {0, 0};   // SM updates per spin since the last re-inversion
   // timing
   bool timing = false;
   Timing tm;
@@ -198,6 +201,9 @@ SlaterArgs slater_args(qmcb_ctx *ctx, int sp) {
   a.gradPR = ctx->gradPR[sp];
   a.lapPR = ctx->lapPR[sp];
   a.singular = ctx->singular[sp];
+  a.dcount = ctx->dcount[sp];
+  a.ubuf = ctx->ubuf[sp];
+  a.zbuf = ctx->zbuf[sp];
   return a;
 }
 
@@ -266,7 +272,7 @@ struct Engine {
     TIMED("basis", (launch_basis<TC, TX>(ctx->dsys, pts, (TX *)ctx->X, ctx->kpad, ctx->ldp,
                                         ctx->stream)));
     const int n = ctx->n[sp];
-    GemmOut go{ctx->vrow, (long)ctx->npad[sp] * ctx->ldv, ctx->ldv, 1, 0, 1, nullptr};
+    GemmOut go{ctx->vrow, (long)ctx->npad[sp], 1, (long)kComp * ctx->npad[sp], 0, 1, nullptr};
     TIMED("orbital_gemm", (launch_orbital_gemm<TX, TO>((const TX *)ctx->Ct[sp], ctx->npad[sp], n,
                                                       (const TX *)ctx->X, ctx->kpad, ctx->ldp,
                                                       (long)num, ctx->kpad, go, ctx->stream)));
@@ -281,8 +287,8 @@ struct Engine {
     a.moved = m;
     a.N = ctx->N;
     a.vrow = ctx->vrow;
-    a.v_s_comp = (long)ctx->npad[sp] * ctx->ldv;
-    a.v_s_orb = ctx->ldv;
+    a.v_s_comp = ctx->npad[sp];
+    a.v_s_walker = (long)kComp * ctx->npad[sp];
     a.n_nuc = ctx->nn;
     a.nuc_xyz = ctx->dsys.nuc_xyz;
     a.nuc_z = ctx->dsys.nuc_z;
@@ -316,9 +322,9 @@ struct Engine {
     return QMCB_OK;
   }
 
-  static qmcb_status refresh(qmcb_ctx *ctx, const int *d_map, int num) {
+  static qmcb_status refresh(qmcb_ctx *ctx, const int *d_map, int num, int spin_mask = 3) {
     for (int sp = 0; sp < 2; sp++) {
-      if (ctx->n[sp] == 0) continue;
+      if (ctx->n[sp] == 0 || !(spin_mask & (1 << sp))) continue;
       SlaterArgs sa = slater_args(ctx, sp);
       TIMED("lu_inverse", launch_lu_inverse<TS>(sa, d_map, num, ctx->stream));
       TIMED("traces_all", launch_traces_all<TS>(sa, d_map, num, ctx->stream));
@@ -432,6 +438,26 @@ qmcb_status create_impl(const qmcb_system *s, const qmcb_options *o, qmcb_ctx *c
   QCHK(upload(ctx, &d.sh_bf0, sh.bf0.data(), sh.bf0.size()));
   QCHK(upload(ctx, &d.sh_ncart, sh.ncart.data(), sh.ncart.size()));
   QCHK(upload(ctx, &d.bf_pow, (const int *)s->bf_pow, 3 * (size_t)ctx->B));
+  {
+    // cost-balanced shell chunks for the basis kernel's second grid dimension
+    const int nch = std::max(1, std::min(32, ctx->nshell / 4));
+    std::vector<double> cost(ctx->nshell);
+    double tot = 0.0;
+    for (int i = 0; i < ctx->nshell; i++) {
+      cost[i] = sh.nprim[i] + 2.0 * sh.ncart[i];
+      tot += cost[i];
+    }
+    std::vector<int> beg(1, 0);
+    double accum = 0.0;
+    for (int i = 0; i < ctx->nshell; i++) {
+      accum += cost[i];
+      if ((int)beg.size() < nch && accum >= tot * beg.size() / nch && i + 1 < ctx->nshell)
+        beg.push_back(i + 1);
+    }
+    beg.push_back(ctx->nshell);
+    d.nchunk = (int)beg.size() - 1;
+    QCHK(upload(ctx, &d.chunk_beg, beg.data(), beg.size()));
+  }
   QCHK(upload(ctx, &d.prim_alpha, s->prim_alpha, (size_t)ctx->nprim));
   QCHK(upload(ctx, &d.prim_coef, s->prim_coef, (size_t)ctx->nprim));
   d.ee[0] = s->ee_updn;
@@ -480,6 +506,9 @@ qmcb_status create_impl(const qmcb_system *s, const qmcb_options *o, qmcb_ctx *c
     QCHK(dalloc(ctx, &ctx->gradPR[sp], W * std::max(n, 1L) * 3));
     QCHK(dalloc(ctx, &ctx->lapPR[sp], W));
     QCHK(dalloc(ctx, &ctx->singular[sp], W));
+    QCHK(dalloc(ctx, &ctx->dcount[sp], W));
+    QCHK(dalloc(ctx, &ctx->ubuf[sp], W * kDelay * std::max(n, 1L)));
+    QCHK(dalloc(ctx, &ctx->zbuf[sp], W * kDelay * std::max(n, 1L)));
   }
   QCHK(dalloc(ctx, &ctx->U, W));
   QCHK(dalloc(ctx, &ctx->Ux, W * N * 3));
@@ -798,8 +827,12 @@ qmcb_status qmcb_vmc_steps(qmcb_ctx *ctx, int32_t num, int64_t first_step, int32
     a.tau = tau;
     st = DISPATCH(ctx, sm(ctx, a, nullptr, num));
     if (st != QMCB_OK) return st;
-    if (ctx->opt.refresh_interval > 0 && (step + 1) % ctx->opt.refresh_interval == 0) {
-      st = DISPATCH(ctx, refresh(ctx, nullptr, num));
+    // FP64 re-inversion of a spin's Slater matrices after refresh_interval
+    // Sherman-Morrison updates of that spin (all walkers move in lockstep)
+    const int sp = m < ctx->n[0] ? 0 : 1;
+    if (ctx->opt.refresh_interval > 0 && ++ctx->sm_updates[sp] >= ctx->opt.refresh_interval) {
+      ctx->sm_updates[sp] = 0;
+      st = DISPATCH(ctx, refresh(ctx, nullptr, num, 1 << sp));
       if (st != QMCB_OK) return st;
     }
   }

@@ -29,6 +29,8 @@ struct DevSystem {
   const double *nuc_xyz, *nuc_z;
   const int *sh_atom, *sh_prim_off, *sh_nprim, *sh_bf0, *sh_ncart;
   const int *bf_pow;  // [B][3]
+  const int *chunk_beg;  // [nchunk+1] shell ranges of the basis kernel's grid.y
+  int nchunk;
   const double *prim_alpha, *prim_coef;
   qmcb_pade ee[3];                    // up-dn, up-up, dn-dn
   const qmcb_pade *en[2];             // [nn] per spin

@@ -58,7 +58,12 @@ struct SlaterArgs {
   double *gradPR;        // [Wc][n][3]
   double *lapPR;         // [Wc]
   int *singular;         // [Wc]
+  // delayed Sherman-Morrison updates: the current inverse is
+  //   A^-1 = inv - sum_{a < dcount} ubuf[a] zbuf[a]^T
+  int *dcount;           // [Wc]
+  double *ubuf, *zbuf;   // [Wc][kDelay][n]
 };
+constexpr int kDelay = 8;
 
 template <typename TS>
 hipError_t launch_lu_inverse(const SlaterArgs &a, const int *walkers, int num, hipStream_t st);
@@ -72,8 +77,8 @@ struct MoveArgs {
   SlaterArgs other;      // the other spin (read only)
   int moved;             // moved electron (global index)
   int N;
-  const void *vrow;      // new rows [5][ldo_orb][Wc] (GEMM output type TO)
-  long v_s_comp, v_s_orb;
+  const void *vrow;      // new rows [batch][5][npad] (GEMM output type TO)
+  long v_s_comp, v_s_walker;
   int n_nuc;
   const double *nuc_xyz, *nuc_z;
   double *R;             // [Wc][N][3] current positions

@@ -18,29 +18,32 @@ __device__ __forceinline__ double ld(const TS *p) { return (double)*p; }
 
 // ---------------------------------------------------------------------------
 // Batched inverse by Gauss-Jordan elimination with partial (row) pivoting,
-// register resident: 256 threads as a 16x16 grid, thread (tr, tc) owns the
-// cyclic block a[tr + 16 i][tc + 16 j], i, j < R (n <= 16R).  Per pivot step
-// only the pivot row and pivot column pass through LDS.  ln|det| and sign
+// register resident: G*G threads as a GxG grid, thread (tr, tc) owns the
+// cyclic block a[tr + G i][tc + G j], i, j < R (n <= G R).  Per pivot step
+// the G owners of the pivot column find the pivot with wavefront shuffles,
+// and only the pivot row and column pass through LDS (two barriers/step).  ln|det| and sign
 // are accumulated from the pivots and row swaps; the LU of the reference
 // (Crout, implicit scaling, long double sums) gives the same inverse and
 // determinant up to rounding.  Singular (an all-zero pivot column) sets the
 // walker's singular flag, as calcOK = false does (QMCSlater.cpp:675-683).
-template <typename TS, int R>
-__global__ void __launch_bounds__(256)
+template <typename TS, int R, int G>
+__global__ void __launch_bounds__(G * G)
 lu_inverse_kernel(SlaterArgs A, const int *walkers, int num) {
   const int b = blockIdx.x;
   if (b >= num) return;
   const int w = walkers ? walkers[b] : b;
   const int n = A.n;
   const int tid = threadIdx.x;
-  const int tr = tid & 15, tc = tid >> 4;  // consecutive lanes -> consecutive rows
-  __shared__ double prow[16 * R], krow[16 * R], fcol[16 * R];
-  __shared__ double cmax[16];
-  __shared__ int cidx[16];
-  __shared__ int ipiv[16 * R];
-  __shared__ int dest[16 * R];
-  __shared__ int cur[16 * R];
-  __shared__ int s_piv, s_sing;
+  // tid = tr + G tc: the G owners of one column are G consecutive lanes
+  const int tr = tid % G, tc = tid / G;
+  constexpr int M = G * R;
+  __shared__ double prow[M], krow[M];
+  __shared__ double fcol[2][M];
+  __shared__ int s_piv[2];
+  __shared__ double s_max[2];
+  __shared__ int ipiv[M];
+  __shared__ int dest[M];
+  __shared__ int cur[M];
 
   const TS *dv = (const TS *)A.dmat + (long)w * kComp * n * n;  // component 0
   double a[R][R];
@@ -48,56 +51,51 @@ lu_inverse_kernel(SlaterArgs A, const int *walkers, int num) {
   for (int i = 0; i < R; i++)
 #pragma unroll
     for (int j = 0; j < R; j++) {
-      const int gi = tr + 16 * i, gj = tc + 16 * j;  // gi: electron, gj: orbital
+      const int gi = tr + G * i, gj = tc + G * j;  // gi: electron, gj: orbital
       double v = (gi == gj) ? 1.0 : 0.0;
       if (gi < n && gj < n) v = ld(dv + (long)gj * n + gi);
       a[i][j] = v;
     }
-  double logdet = 0.0, sgn = 1.0;
-  if (tid == 0) s_sing = 0;
+  // |det| kept as mantissa * 2^expo to stay in range
+  double mant = 1.0, sgn = 1.0;
+  int expo = 0;
+  bool sing = false;
   for (int k = 0; k < n; k++) {
-    // ---- pivot search in column k, rows >= k
-    const int kj = k >> 4, kt = k & 15;
+    const int kj = k / G, kt = k % G, buf = k & 1;
+    // ---- pivot search in column k (rows >= k) by its 16 owner lanes
     if (tc == kt) {
       double best = -1.0;
-      int bi = -1;
+      int bi = 0x7fffffff;
 #pragma unroll
       for (int i = 0; i < R; i++) {
-        const int gi = tr + 16 * i;
+        const int gi = tr + G * i;
         double aik = 0.0;
 #pragma unroll
         for (int j = 0; j < R; j++) aik = (j == kj) ? a[i][j] : aik;
+        fcol[buf][gi] = aik;
         if (gi >= k && gi < n) {
           const double v = fabs(aik);
           if (v > best) { best = v; bi = gi; }
         }
       }
-      cmax[tr] = best;
-      cidx[tr] = bi;
+#pragma unroll
+      for (int off = G / 2; off >= 1; off >>= 1) {
+        const double ob = __shfl_xor(best, off, 64);
+        const int oi = __shfl_xor(bi, off, 64);
+        if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+      }
+      if (tr == 0) { s_piv[buf] = bi; s_max[buf] = best; }
     }
     __syncthreads();
-    if (tid == 0) {
-      double best = -1.0;
-      int bi = -1;
-      for (int i = 0; i < 16; i++)
-        if (cidx[i] >= 0 && (cmax[i] > best || (cmax[i] == best && cidx[i] < bi))) {
-          best = cmax[i]; bi = cidx[i];
-        }
-      s_piv = bi;
-      ipiv[k] = bi;
-      if (!(best > 0.0)) s_sing = 1;
-    }
-    __syncthreads();
-    if (s_sing) break;
-    const int p = s_piv;
-    // ---- rows p and k to LDS
-    const int pi = p >> 4;
-    if (tr == (p & 15)) {
+    if (!(s_max[buf] > 0.0)) { sing = true; break; }
+    const int p = s_piv[buf], pi = p / G;
+    if (tid == 0) ipiv[k] = p;
+    if (tr == p % G) {
 #pragma unroll
       for (int i = 0; i < R; i++)
         if (i == pi) {
 #pragma unroll
-          for (int j = 0; j < R; j++) prow[tc + 16 * j] = a[i][j];
+          for (int j = 0; j < R; j++) prow[tc + G * j] = a[i][j];
         }
     }
     if (tr == kt && p != k) {
@@ -105,7 +103,7 @@ lu_inverse_kernel(SlaterArgs A, const int *walkers, int num) {
       for (int i = 0; i < R; i++)
         if (i == kj) {
 #pragma unroll
-          for (int j = 0; j < R; j++) krow[tc + 16 * j] = a[i][j];
+          for (int j = 0; j < R; j++) krow[tc + G * j] = a[i][j];
         }
     }
     __syncthreads();
@@ -115,49 +113,40 @@ lu_inverse_kernel(SlaterArgs A, const int *walkers, int num) {
         for (int i = 0; i < R; i++)
           if (i == kj) {
 #pragma unroll
-            for (int j = 0; j < R; j++) a[i][j] = prow[tc + 16 * j];
+            for (int j = 0; j < R; j++) a[i][j] = prow[tc + G * j];
           }
       }
-      if (tr == (p & 15)) {
+      if (tr == p % G) {
 #pragma unroll
         for (int i = 0; i < R; i++)
           if (i == pi) {
 #pragma unroll
-            for (int j = 0; j < R; j++) a[i][j] = krow[tc + 16 * j];
+            for (int j = 0; j < R; j++) a[i][j] = krow[tc + G * j];
           }
       }
       sgn = -sgn;
     }
     const double piv = prow[k];
-    logdet += log(fabs(piv));
     if (piv < 0) sgn = -sgn;
+    int e2;
+    mant = frexp(mant * fabs(piv), &e2);
+    expo += e2;
     const double ip = 1.0 / piv;
-    if (tc == kt) {
-#pragma unroll
-      for (int i = 0; i < R; i++) {
-        double aik = 0.0;
-#pragma unroll
-        for (int j = 0; j < R; j++) aik = (j == kj) ? a[i][j] : aik;
-        fcol[tr + 16 * i] = aik;
-      }
-    }
-    __syncthreads();
-    // ---- elimination
+    // ---- elimination: row k scaled, other rows reduced (in-place Gauss-Jordan)
 #pragma unroll
     for (int i = 0; i < R; i++) {
-      const int gi = tr + 16 * i;
-      const double f = fcol[gi];
+      const int gi = tr + G * i;
+      // column-k entry of row gi after the interchange
+      const double f = (gi == p) ? fcol[buf][k] : fcol[buf][gi];
 #pragma unroll
       for (int j = 0; j < R; j++) {
-        const int gj = tc + 16 * j;
+        const int gj = tc + G * j;
         const double pr = (gj == k) ? ip : prow[gj] * ip;
         if (gi == k) a[i][j] = pr;
         else a[i][j] = (gj == k) ? -f * ip : a[i][j] - f * pr;
       }
     }
-    __syncthreads();
   }
-  const bool sing = s_sing != 0;
   if (tid == 0) {
     // undo the row interchanges as column interchanges, in reverse order
     for (int c = 0; c < n; c++) dest[c] = c;
@@ -177,29 +166,182 @@ lu_inverse_kernel(SlaterArgs A, const int *walkers, int num) {
     for (int i = 0; i < R; i++)
 #pragma unroll
       for (int j = 0; j < R; j++) {
-        const int gi = tr + 16 * i, gj = tc + 16 * j;
+        const int gi = tr + G * i, gj = tc + G * j;
         if (gi < n && gj < n) inv[(long)gi * n + dest[gj]] = a[i][j];
       }
   }
   if (tid == 0) {
-    A.logdet[w] = sing ? 0.0 : logdet;
+    A.logdet[w] = sing ? 0.0 : log(mant) + expo * 0.6931471805599453;
     A.sgn[w] = sing ? 0.0 : sgn;
     A.singular[w] = sing ? 1 : 0;
+    A.dcount[w] = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Row-resident Gauss-Jordan for NP in {64, 128}: row i of the matrix lives in
+// the registers of the lane pair (2i, 2i+1), NP/2 doubles each (half h holds
+// register positions h*NP/2 ...).  Pivot columns are processed in blocks of 16
+// with compile-time register indices; after each block every row is rotated
+// left by 16 positions (16 values cross between the partner lanes by
+// shuffle), so the next block's pivot columns are again positions 0..15 of
+// half 0 and no register array is indexed dynamically.  All rows share the
+// rotation state, so LDS rows (pivot row, swapped row) are exchanged in
+// position order.  Per pivot: a wavefront argmax of |a[k]| over rows >= k,
+// one row interchange through LDS, NP/2 FMAs per lane.  The inverse is written
+// through an LDS tile so that the column permutation of the interchanges
+// lands in coalesced stores of rows of A^-1(o, e).
+template <typename TS, int NP>
+__global__ void __launch_bounds__(2 * NP)
+gj_rows_kernel(SlaterArgs A, const int *walkers, int num) {
+  constexpr int KB = 16, NB = NP / KB, H = NP / 2, NT = 2 * NP, NWV = NT / 64, TR = 32;
+  const int b = blockIdx.x;
+  if (b >= num) return;
+  const int w = walkers ? walkers[b] : b;
+  const int n = A.n;
+  const int t = threadIdx.x, i = t >> 1, h = t & 1;
+  __shared__ double prow[NP], krow[NP];
+  __shared__ double wmax[NWV];
+  __shared__ int widx[NWV];
+  __shared__ int ipiv[NP], dest[NP], cur[NP];
+  __shared__ double tile[TR][NP + 1];
+
+  const TS *dv = (const TS *)A.dmat + (long)w * kComp * n * n;  // value matrix, (o, e)
+  double a[H];
+#pragma unroll
+  for (int j = 0; j < H; j++) {
+    const int col = h * H + j;
+    a[j] = (i < n && col < n) ? ld(dv + (long)col * n + i) : (i == col ? 1.0 : 0.0);
+  }
+  double mant = 1.0, sgn = 1.0;
+  int expo = 0;
+  bool sing = false;
+  for (int kb = 0; kb < NB; kb++) {
+#pragma unroll
+    for (int kk = 0; kk < KB; kk++) {
+      const int k = kb * KB + kk;
+      if (k < n && !sing) {
+        const double akk = __shfl_xor(a[kk], h, 64);  // position kk of half 0
+        double best = (h == 0 && i >= k && i < n) ? fabs(akk) : -1.0;
+        int bi = i;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+          const double ob = __shfl_xor(best, off, 64);
+          const int oi = __shfl_xor(bi, off, 64);
+          if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+        }
+        if ((t & 63) == 0) { wmax[t >> 6] = best; widx[t >> 6] = bi; }
+        __syncthreads();
+        best = wmax[0];
+        bi = widx[0];
+#pragma unroll
+        for (int q = 1; q < NWV; q++)
+          if (wmax[q] > best || (wmax[q] == best && widx[q] < bi)) { best = wmax[q]; bi = widx[q]; }
+        if (!(best > 0.0)) {
+          sing = true;
+        } else {
+          const int p = bi;
+          if (i == p) {
+#pragma unroll
+            for (int j = 0; j < H; j++) prow[h * H + j] = a[j];
+          }
+          if (i == k && p != k) {
+#pragma unroll
+            for (int j = 0; j < H; j++) krow[h * H + j] = a[j];
+          }
+          if (t == 0) ipiv[k] = p;
+          __syncthreads();
+          if (p != k) {
+            if (i == k) {
+#pragma unroll
+              for (int j = 0; j < H; j++) a[j] = prow[h * H + j];
+            } else if (i == p) {
+#pragma unroll
+              for (int j = 0; j < H; j++) a[j] = krow[h * H + j];
+            }
+            sgn = -sgn;
+          }
+          const double piv = prow[kk];
+          if (piv < 0) sgn = -sgn;
+          int e2;
+          mant = frexp(mant * fabs(piv), &e2);
+          expo += e2;
+          const double ip = 1.0 / piv;
+          const double f = __shfl_xor(a[kk], h, 64);  // after the interchange
+          if (i == k) {
+#pragma unroll
+            for (int j = 0; j < H; j++) a[j] = (h == 0 && j == kk) ? ip : prow[h * H + j] * ip;
+          } else {
+            const double fi = f * ip;
+#pragma unroll
+            for (int j = 0; j < H; j++)
+              a[j] = (h == 0 && j == kk) ? -fi : a[j] - fi * prow[h * H + j];
+          }
+        }
+        __syncthreads();
+      }
+    }
+    // rotate every row left by KB positions: half 0 takes its own [KB, H)
+    // plus half 1's [0, KB); half 1 takes its own [KB, H) plus half 0's [0, KB)
+    double tq[KB];
+#pragma unroll
+    for (int q = 0; q < KB; q++) tq[q] = __shfl_xor(a[q], 1, 64);
+#pragma unroll
+    for (int j = 0; j < H - KB; j++) a[j] = a[j + KB];
+#pragma unroll
+    for (int q = 0; q < KB; q++) a[H - KB + q] = tq[q];
+  }
+  if (t == 0) {
+    for (int c = 0; c < n; c++) dest[c] = c;
+    if (!sing) {
+      for (int c = 0; c < n; c++) cur[c] = c;
+      for (int k = n - 1; k >= 0; k--) {
+        const int q = ipiv[k];
+        if (q != k) { const int tt = cur[k]; cur[k] = cur[q]; cur[q] = tt; }
+      }
+      for (int c = 0; c < n; c++) dest[cur[c]] = c;
+    }
+  }
+  __syncthreads();
+  // row o = i of A^-1; register position j of half h is column h*H + j,
+  // which belongs to electron dest[h*H + j]
+  double *inv = A.inv + (long)w * n * n;
+  if (!sing) {
+    for (int r0 = 0; r0 < n; r0 += TR) {
+      if (i >= r0 && i < r0 + TR) {
+#pragma unroll
+        for (int j = 0; j < H; j++) {
+          const int col = h * H + j;
+          if (col < n) tile[i - r0][dest[col]] = a[j];
+        }
+      }
+      __syncthreads();
+      for (int idx = t; idx < TR * n; idx += NT) {
+        const int r = idx / n, e = idx % n;
+        if (r0 + r < n) inv[(long)(r0 + r) * n + e] = tile[r][e];
+      }
+      __syncthreads();
+    }
+  }
+  if (t == 0) {
+    A.logdet[w] = sing ? 0.0 : log(mant) + expo * 0.6931471805599453;
+    A.sgn[w] = sing ? 0.0 : sgn;
+    A.singular[w] = sing ? 1 : 0;
+    A.dcount[w] = 0;
   }
 }
 
 template <typename TS>
 hipError_t launch_lu_inverse(const SlaterArgs &a, const int *walkers, int num, hipStream_t st) {
   if (num <= 0 || a.n <= 0) return hipSuccess;
-  if (a.n > 256) return hipErrorInvalidValue;
   if (a.n <= 16)
-    hipLaunchKernelGGL((lu_inverse_kernel<TS, 1>), dim3(num), dim3(256), 0, st, a, walkers, num);
+    hipLaunchKernelGGL((lu_inverse_kernel<TS, 1, 16>), dim3(num), dim3(256), 0, st, a, walkers, num);
   else if (a.n <= 32)
-    hipLaunchKernelGGL((lu_inverse_kernel<TS, 2>), dim3(num), dim3(256), 0, st, a, walkers, num);
+    hipLaunchKernelGGL((lu_inverse_kernel<TS, 2, 16>), dim3(num), dim3(256), 0, st, a, walkers, num);
   else if (a.n <= 64)
-    hipLaunchKernelGGL((lu_inverse_kernel<TS, 4>), dim3(num), dim3(256), 0, st, a, walkers, num);
+    hipLaunchKernelGGL((gj_rows_kernel<TS, 64>), dim3(num), dim3(128), 0, st, a, walkers, num);
   else if (a.n <= 128)
-    hipLaunchKernelGGL((lu_inverse_kernel<TS, 8>), dim3(num), dim3(256), 0, st, a, walkers, num);
+    hipLaunchKernelGGL((gj_rows_kernel<TS, 128>), dim3(num), dim3(256), 0, st, a, walkers, num);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();
@@ -279,118 +421,137 @@ __device__ inline double drift_factor_unequal(const double g[3], const double x[
 }
 
 // ---------------------------------------------------------------------------
-// Single-electron move of spin s for one walker per workgroup.
+// Single-electron move of spin s, one walker per workgroup, one streaming pass.
 //
-// NT threads = NMAX columns x SL row slices; thread (c, sl) keeps rows
-// [sl*RPT, (sl+1)*RPT) of A^-1 column c in registers for the whole kernel:
-//   1. w(c) = v . A^-1(:,c) (v = new orbital row), ratio = w(r)
-//   2. A'^-1(:,c) = A^-1(:,c) - A^-1(:,r) w(c)/ratio (c != r), A^-1(:,r)/ratio
-//      -- the same row-replacement formula as inverseUpdateOneRow
-//   3. traces at the proposal: gradPR'(c) and lap, streaming dmat column c
-//      (new row r taken from v)
-//   4. assembly over all N electrons: grad ln Psi', Laplacian, E_L', drift
-//   5. stage the proposal (mode 0) or Metropolis-accept it and commit A'^-1
-//      in place (mode 1: device VMC).
-// HBM traffic per move: A^-1 read once (+ write when accepted), four
-// derivative matrices read once; everything else is O(N).
-template <typename TS, typename TO, int NMAX, int NT>
+// With w(c) = v . A^-1(:,c) (v = the moved electron's new orbital row) and
+// ratio = w(r), the row-replacement update (inverseUpdateOneRow) is
+//   A'^-1 = A^-1 - A^-1(:,r) z^T,   z(c) = (w(c) - delta_cr) / ratio,
+// so every trace at the proposal splits into two sums over the current
+// inverse:
+//   sum_o d_k D(c,o) A'^-1(o,c) = T1_k(c) - (w(c)/ratio) T2_k(c),
+//   T1_k(c) = sum_o d_k D(c,o) A^-1(o,c),  T2_k(c) = sum_o d_k D(c,o) A^-1(o,r).
+//
+// Delayed updates: the resident inverse is kept as A_0 - sum_a u_a z_a^T with
+// up to kDelay pending rank-1 terms (u_a = column r_a of the inverse at update
+// a).  An accepted move only appends (u, z) -- 2n doubles -- instead of
+// rewriting the n x n inverse; the pending terms are folded into A_0 during
+// the streaming pass of the move that finds the buffer full.
+//
+// Pass 1 streams A_0 and the four derivative matrices once, forms the current
+// inverse element on the fly and accumulates w, T1, T2 per column (9 registers).
+// Thread mapping: c = t % NC (electron column), sl = t / NC (row slice).
+template <typename TS, typename TO, int NC, int NT>
 __global__ void __launch_bounds__(NT)
 sm_move_kernel(MoveArgs A, const int *walkers, int num) {
-  constexpr int SL = NT / NMAX;
-  constexpr int RPT = NMAX / SL;
+  constexpr int SL = NT / NC;
   constexpr int NW = NT / 64;
+  constexpr int KD = kDelay;
   const int b = blockIdx.x;
   if (b >= num) return;
   const int w = walkers ? walkers[b] : b;
-  const int t = threadIdx.x, c = t % NMAX, sl = t / NMAX;
+  const int t = threadIdx.x, c = t % NC, sl = t / NC;
   const int n = A.s.n, N = A.N, r = A.moved - A.s.spin_start;
   const long nn2 = (long)n * n;
+  const int rows = (n + SL - 1) / SL;
+  const int o_beg = sl * rows, o_end = min(n, o_beg + rows);
+  const int dc = A.s.dcount[w];
+  const bool flush = (A.mode == 1) && dc >= KD;
 
-  __shared__ double v[kComp][NMAX];
-  __shared__ double red[4][SL][NMAX];
-  __shared__ double colr[NMAX];
-  __shared__ double wv[NMAX];
-  __shared__ double gq[4][NMAX];
-  __shared__ double wred[4][NW];
+  __shared__ double v[kComp][NC];
+  __shared__ double colr[NC];
+  __shared__ double ub[KD][NC];
+  __shared__ double zr[KD];
+  __shared__ double red[9][SL > 1 ? SL - 1 : 1][NC];
+  __shared__ double wv[NC];
+  __shared__ double gq[4][NC];
+  __shared__ double wred5[5][NW];
   __shared__ double s_bc[8];
 
-  const TO *vr = (const TO *)A.vrow;
+  const TO *vr = (const TO *)A.vrow + (long)b * A.v_s_walker;
+  double *inv = A.s.inv + (long)w * nn2;
+  const double *ubw = A.s.ubuf + (long)w * KD * n;
+  const double *zbw = A.s.zbuf + (long)w * KD * n;
   for (int i = t; i < kComp * n; i += NT) {
     const int k = i / n, o = i % n;
-    v[k][o] = (double)vr[(long)k * A.v_s_comp + (long)o * A.v_s_orb + b];
+    v[k][o] = (double)vr[(long)k * A.v_s_comp + o];
+  }
+  for (int i = t; i < KD * NC; i += NT) {
+    const int a = i / NC, o = i % NC;
+    ub[a][o] = (a < dc && o < n) ? ubw[(long)a * n + o] : 0.0;
+  }
+  if (t < KD) zr[t] = t < dc ? zbw[(long)t * n + r] : 0.0;
+  double zc[KD];
+#pragma unroll
+  for (int a = 0; a < KD; a++) zc[a] = (a < dc && c < n) ? zbw[(long)a * n + c] : 0.0;
+  __syncthreads();
+  for (int o = t; o < n; o += NT) {
+    double x = inv[(long)o * n + r];
+#pragma unroll
+    for (int a = 0; a < KD; a++) x -= ub[a][o] * zr[a];
+    colr[o] = x;
   }
   __syncthreads();
 
-  // ---- phase 1: w(c)
-  const double *inv = A.s.inv + (long)w * nn2;
-  double a[RPT];
-  double wp = 0.0;
+  // ---- pass 1
+  const TS *dm = (const TS *)A.s.dmat + (long)w * kComp * nn2;
+  double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // w, T1[4], T2[4]
+  if (c < n) {
+    double *ip = inv + c;
+    const TS *d1 = dm + 1 * nn2 + c, *d2 = dm + 2 * nn2 + c, *d3 = dm + 3 * nn2 + c,
+             *d4 = dm + 4 * nn2 + c;
+#pragma unroll 4
+    for (int o = o_beg; o < o_end; o++) {
+      const long off = (long)o * n;
+      double a = ip[off];
+      const double x = ld(d1 + off), y = ld(d2 + off), z = ld(d3 + off), l = ld(d4 + off);
 #pragma unroll
-  for (int i = 0; i < RPT; i++) {
-    const int o = sl * RPT + i;
-    a[i] = (o < n && c < n) ? inv[(long)o * n + c] : 0.0;
-    if (o < n) wp += v[0][o] * a[i];
-  }
-  red[0][sl][c] = wp;
-  if (c == r) {
-#pragma unroll
-    for (int i = 0; i < RPT; i++) {
-      const int o = sl * RPT + i;
-      if (o < n) colr[o] = a[i];
+      for (int q = 0; q < KD; q++) a -= ub[q][o] * zc[q];
+      if (flush) ip[off] = a;
+      const double cr = colr[o];
+      acc[0] += v[0][o] * a;
+      acc[1] += x * a; acc[2] += y * a; acc[3] += z * a; acc[4] += l * a;
+      acc[5] += x * cr; acc[6] += y * cr; acc[7] += z * cr; acc[8] += l * cr;
     }
   }
-  __syncthreads();
-  if (sl == 0 && c < n) {
-    double s = 0.0;
+  if (sl > 0) {
 #pragma unroll
-    for (int q = 0; q < SL; q++) s += red[0][q][c];
-    wv[c] = s;
+    for (int k = 0; k < 9; k++) red[k][sl - 1][c] = acc[k];
+  }
+  __syncthreads();
+  if (sl == 0) {
+#pragma unroll
+    for (int q = 1; q < SL; q++)
+#pragma unroll
+      for (int k = 0; k < 9; k++) acc[k] += red[k][q - 1][c];
+    wv[c] = c < n ? acc[0] : 0.0;
+  }
+  // new derivative row of electron r times the new column r: v_k . colr / ratio
+  if (t < 64) {
+    double dv[4] = {0, 0, 0, 0};
+    for (int o = t; o < n; o += 64)
+#pragma unroll
+      for (int k = 0; k < 4; k++) dv[k] += v[1 + k][o] * colr[o];
+#pragma unroll
+    for (int k = 0; k < 4; k++) dv[k] = wave_sum(dv[k]);
+    if (t == 0)
+#pragma unroll
+      for (int k = 0; k < 4; k++) s_bc[4 + k] = dv[k];
   }
   __syncthreads();
   const double ratio = wv[r];
   const bool ok = (ratio != 0.0) && isfinite(ratio);
   const double qinv = 1.0 / ratio;
-  const double wc = c < n ? wv[c] * qinv : 0.0;
-
-  // ---- phase 2: A'^-1 in registers
-#pragma unroll
-  for (int i = 0; i < RPT; i++) {
-    const int o = sl * RPT + i;
-    if (o < n) a[i] = (c == r) ? a[i] * qinv : a[i] - wc * colr[o];
-  }
-
-  // ---- phase 3: traces at the proposal
-  const TS *dm = (const TS *)A.s.dmat + (long)w * kComp * nn2;
-  double gp[4] = {0.0, 0.0, 0.0, 0.0};
-  if (c < n) {
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      const TS *col = dm + (long)(1 + k) * nn2 + c;
-#pragma unroll
-      for (int i = 0; i < RPT; i++) {
-        const int o = sl * RPT + i;
-        if (o < n) {
-          const double d = (c == r) ? v[1 + k][o] : ld(col + (long)o * n);
-          gp[k] += d * a[i];
-        }
-      }
-    }
-  }
-#pragma unroll
-  for (int k = 0; k < 4; k++) red[k][sl][c] = gp[k];
-  __syncthreads();
   if (sl == 0) {
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-      double s = 0.0;
-#pragma unroll
-      for (int q = 0; q < SL; q++) s += red[k][q][c];
-      gq[k][c] = (c < n) ? s : 0.0;
+      double g = 0.0;
+      if (c < n) g = (c == r) ? s_bc[4 + k] * qinv : acc[1 + k] - wv[c] * qinv * acc[5 + k];
+      gq[k][c] = g;
     }
   }
   __syncthreads();
 
-  // ---- phase 4: assembly over all electrons
+  // ---- assembly over all electrons
   const int so = A.other.spin_start, no = A.other.n, ss = A.s.spin_start;
   const double *pUx = A.pUx + (long)b * N * 3;
   const double *cUx = A.Ux + (long)w * N * 3;
@@ -437,9 +598,7 @@ sm_move_kernel(MoveArgs A, const int *walkers, int num) {
       }
     }
   }
-  // block reduction of lsum, fwd, rev, lapc (+gsq for the equal-electron drift)
   double vals[5] = {lsum, fwd, rev, lapc, gsq};
-  __shared__ double wred5[5][NW];
 #pragma unroll
   for (int k = 0; k < 5; k++) {
     const double s = wave_sum(vals[k]);
@@ -447,13 +606,11 @@ sm_move_kernel(MoveArgs A, const int *walkers, int num) {
   }
   __syncthreads();
   if (t == 0) {
-    double tot[5];
     for (int k = 0; k < 5; k++) {
       double s = 0.0;
       for (int q = 0; q < NW; q++) s += wred5[k][q];
-      tot[k] = s;
+      s_bc[k] = s;
     }
-    s_bc[0] = tot[0]; s_bc[1] = tot[1]; s_bc[2] = tot[2]; s_bc[3] = tot[3]; s_bc[4] = tot[4];
   }
   __syncthreads();
   const double lap_s = s_bc[3];
@@ -468,7 +625,6 @@ sm_move_kernel(MoveArgs A, const int *walkers, int num) {
   bool sing = !ok || A.s.singular[w] || (no > 0 && A.other.singular[w]);
   if (!(s_bc[4] > 0.0) || !isfinite(s_bc[4])) sing = true;
 
-  // modified drift output (staged/evaluate outputs)
   if (A.out.mod_drift) {
     double feq = 1.0;
     if (A.drift_mod == QMCB_DRIFT_UMRIGAR93_EQUAL) {
@@ -505,18 +661,10 @@ sm_move_kernel(MoveArgs A, const int *walkers, int num) {
   }
 
   if (A.mode == 0) {
-    // ---- stage the proposal
-    if (c < n) {
-      double *st = A.st_inv + (long)b * nn2;
-#pragma unroll
-      for (int i = 0; i < RPT; i++) {
-        const int o = sl * RPT + i;
-        if (o < n) st[(long)o * n + c] = a[i];
-      }
-      if (sl == 0) {
-        double *g = A.st_grad + ((long)b * n + c) * 3;
-        g[0] = gq[0][c]; g[1] = gq[1][c]; g[2] = gq[2][c];
-      }
+    // ---- stage the proposal: A'^-1 = A^-1 - colr z^T written in full
+    if (sl == 0 && c < n) {
+      double *g = A.st_grad + ((long)b * n + c) * 3;
+      g[0] = gq[0][c]; g[1] = gq[1][c]; g[2] = gq[2][c];
     }
     if (t == 0) {
       A.st_lap[b] = lap_s;
@@ -524,31 +672,40 @@ sm_move_kernel(MoveArgs A, const int *walkers, int num) {
       A.st_sgn[b] = sgn_s_new;
       A.st_singular[b] = sing ? 1 : 0;
     }
+    if (ok && c < n) {
+      const double zcur = (wv[c] - (c == r ? 1.0 : 0.0)) * qinv;
+      const double *ip = inv + c;
+      double *op = A.st_inv + (long)b * nn2 + c;
+      for (int o = o_beg; o < o_end; o++) {
+        double a = ip[(long)o * n];
+#pragma unroll
+        for (int q = 0; q < KD; q++) a -= ub[q][o] * zc[q];
+        op[(long)o * n] = a - colr[o] * zcur;
+      }
+    }
     return;
   }
 
   // ---- mode 1: Metropolis with the Green's function ratio
-  // (QMCWalker.cpp:1578-1685); p = |Psi'/Psi|^2 G(R<-R')/G(R'<-R)
+  // (QMCWalker.cpp:1578-1685): p = |Psi'/Psi|^2 G(R<-R')/G(R'<-R)
   const double logpsi_old = ld_s + ld_o + A.U[w];
-  double p = sing ? 0.0 : exp(2.0 * (logpsi_new - logpsi_old) - (s_bc[2] - s_bc[1]) / (2.0 * tau));
+  double p = sing ? 0.0
+                  : exp(2.0 * (logpsi_new - logpsi_old) - (s_bc[2] - s_bc[1]) / (2.0 * tau));
   if (!(p == p)) p = 0.0;
   if (p > 1.0) p = 1.0;
   const bool accept = p > A.unif[b];
-  __syncthreads();  // all reads of the current state are done
+  __syncthreads();  // every read of the current state is done
+  const int base = flush ? 0 : dc;
   if (accept) {
-    double *invw = A.s.inv + (long)w * nn2;
-    if (c < n) {
-#pragma unroll
-      for (int i = 0; i < RPT; i++) {
-        const int o = sl * RPT + i;
-        if (o < n) invw[(long)o * n + c] = a[i];
-      }
-      if (sl == 0) {
-        double *g = A.s.gradPR + ((long)w * n + c) * 3;
-        g[0] = gq[0][c]; g[1] = gq[1][c]; g[2] = gq[2][c];
-      }
+    // append the rank-1 term of this update to the delay buffer
+    double *ubo = A.s.ubuf + ((long)w * KD + base) * n;
+    double *zbo = A.s.zbuf + ((long)w * KD + base) * n;
+    for (int o = t; o < n; o += NT) ubo[o] = colr[o];
+    if (sl == 0 && c < n) {
+      zbo[c] = (wv[c] - (c == r ? 1.0 : 0.0)) * qinv;
+      double *g = A.s.gradPR + ((long)w * n + c) * 3;
+      g[0] = gq[0][c]; g[1] = gq[1][c]; g[2] = gq[2][c];
     }
-    // new row r of D and its derivatives -> column r of the stored matrices
     TS *dmw = (TS *)A.s.dmat + (long)w * kComp * nn2;
     for (int i = t; i < kComp * n; i += NT) {
       const int k = i / n, o = i % n;
@@ -557,6 +714,7 @@ sm_move_kernel(MoveArgs A, const int *walkers, int num) {
     double *Uxw = A.Ux + (long)w * N * 3;
     for (int i = t; i < 3 * N; i += NT) Uxw[i] = pUx[i];
     if (t == 0) {
+      A.s.dcount[w] = base + 1;
       A.s.lapPR[w] = lap_s;
       A.s.logdet[w] = ld_s + log(fabs(ratio));
       A.s.sgn[w] = sgn_s_new;
@@ -568,6 +726,8 @@ sm_move_kernel(MoveArgs A, const int *walkers, int num) {
       A.Ven[w] = A.pVen[b];
       A.cur_el[w] = EL;
     }
+  } else if (flush && t == 0) {
+    A.s.dcount[w] = 0;
   }
   if (t == 0) {
     const double el = accept ? EL : A.cur_el[w];
@@ -593,7 +753,7 @@ hipError_t launch_sm_move(const MoveArgs &a, const int *walkers, int num, hipStr
     hipLaunchKernelGGL((sm_move_kernel<TS, TO, 64, 256>), dim3(num), dim3(256), 0, st, a,
                        walkers, num);
   else if (n <= 128)
-    hipLaunchKernelGGL((sm_move_kernel<TS, TO, 128, 512>), dim3(num), dim3(512), 0, st, a,
+    hipLaunchKernelGGL((sm_move_kernel<TS, TO, 128, 256>), dim3(num), dim3(256), 0, st, a,
                        walkers, num);
   else
     return hipErrorInvalidValue;
@@ -622,12 +782,14 @@ commit_kernel(MoveArgs A, const int *walkers, const uint8_t *mask, int num) {
   TS *dmw = (TS *)A.s.dmat + (long)w * kComp * nn2;
   for (int i = t; i < kComp * n; i += blockDim.x) {
     const int k = i / n, o = i % n;
-    dmw[(long)k * nn2 + (long)o * n + r] = (TS)(double)vr[(long)k * A.v_s_comp + (long)o * A.v_s_orb + b];
+    dmw[(long)k * nn2 + (long)o * n + r] =
+        (TS)(double)vr[(long)b * A.v_s_walker + (long)k * A.v_s_comp + o];
   }
   double *Uxw = A.Ux + (long)w * N * 3;
   const double *pUx = A.pUx + (long)b * N * 3;
   for (int i = t; i < 3 * N; i += blockDim.x) Uxw[i] = pUx[i];
   if (t == 0) {
+    A.s.dcount[w] = 0;
     A.s.lapPR[w] = A.st_lap[b];
     A.s.logdet[w] = A.st_logdet[b];
     A.s.sgn[w] = A.st_sgn[b];
