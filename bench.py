@@ -89,6 +89,48 @@ def run_path(s, precision, W, steps, warmup, tau, block, refresh, seed):
     }
 
 
+def accuracy_report(s, n_conf=10000, seed=4321, tau=0.01, sweeps=1, precision="fp32"):
+    """Per-quantity error of the mixed path against the FP64 path on n_conf
+    fixed configurations (FP64 VMC positions after `sweeps` sweeps from
+    N(R_A, 0.5^2)).  The FP64 path itself matches the reference evaluate to
+    <= 1e-10 on the golden configurations.  Both all-electron evaluates and a
+    single-electron proposal (moved electron 0, Sherman-Morrison on FP32 rows)
+    are compared."""
+    from qmcbeaver_amd import synth, Context
+    N = s.n_elec
+    R0 = synth.initial_walkers(s, n_conf, seed)
+    ids = np.arange(n_conf)
+    with Context(s.pack(), precision="fp64", max_walkers=n_conf, seed=seed) as c64:
+        c64.evaluate(ids, R0)
+        if sweeps:
+            c64.vmc_steps(n_conf, 0, sweeps * N, tau)
+        R = c64.positions(n_conf)
+        a64 = c64.evaluate(ids, R)
+        Rp = R.copy()
+        Rp[:, 0, :] += 0.05
+        p64 = c64.evaluate(ids, Rp, whichE=0)
+    with Context(s.pack(), precision=precision, max_walkers=n_conf, seed=seed) as c32:
+        a32 = c32.evaluate(ids, R)
+        p32 = c32.evaluate(ids, Rp, whichE=0)
+    ok = ~(a64["singular"].astype(bool) | a32["singular"].astype(bool) |
+           p64["singular"].astype(bool) | p32["singular"].astype(bool))
+
+    def stats(x, y, per=1.0):
+        d = np.abs(np.asarray(x)[ok] - np.asarray(y)[ok]).reshape(ok.sum(), -1)
+        d = d.max(axis=1) / per
+        return {"max": float(d.max()), "rms": float(np.sqrt(np.mean(d * d))),
+                "p99": float(np.percentile(d, 99))}
+    out = {"configurations": int(ok.sum()), "reference": "FP64 path", "precision": precision,
+           "all_electron": {"log_psi": stats(a32["log_psi"], a64["log_psi"]),
+                            "grad_lnpsi": stats(a32["grad_lnpsi"], a64["grad_lnpsi"]),
+                            "e_local_per_electron": stats(a32["e_local"], a64["e_local"], N)},
+           "single_electron": {"log_psi": stats(p32["log_psi"], p64["log_psi"]),
+                               "grad_lnpsi": stats(p32["grad_lnpsi"], p64["grad_lnpsi"]),
+                               "e_local_per_electron": stats(p32["e_local"], p64["e_local"], N)},
+           "sign_mismatch": int(np.sum(a32["sign_psi"][ok] != a64["sign_psi"][ok]))}
+    return out
+
+
 def cpu_baseline(s, tau, seed, target_s=15.0):
     """The reference's own evaluate (oracle/_ref, compiled from its sources)
     timed on this host's cores, 1 thread; else the C oracle port."""
@@ -139,9 +181,11 @@ def main():
     ap.add_argument("--tau", type=float, default=0.01)
     ap.add_argument("--block", type=int, default=16, help="steps per timed block")
     ap.add_argument("--refresh", type=int, default=64)
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp32store", "fp64"])
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--accuracy", type=int, default=10000,
+                    help="configurations for the mixed-vs-FP64 error report (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: the walker-evaluation path "
@@ -154,9 +198,13 @@ def main():
     mixed = run_path(s, args.precision, args.walkers, args.steps, args.warmup, args.tau,
                      args.block, args.refresh if args.precision != "fp64" else 0, 1234)
     fp64 = None
+    store = None
     if not args.no_fp64 and args.precision != "fp64":
         fp64 = run_path(s, "fp64", args.walkers, args.steps, args.warmup, args.tau, args.block,
                         0, 1234)
+        if args.precision == "fp32":
+            store = run_path(s, "fp32store", args.walkers, args.steps, args.warmup, args.tau,
+                             args.block, args.refresh, 1234)
 
     s_d = 8 if args.precision == "fp64" else 4
     kt = mixed["kernels"].get("sm_move", {"ms": float("nan"), "launches": 1})
@@ -208,6 +256,20 @@ def main():
     if fp64 is not None:
         out["fp64"]["kernels_ms_per_step"] = {k: v["ms"] / max(args.steps, 1)
                                               for k, v in fp64["kernels"].items()}
+    if store is not None:
+        out["fp32store"] = {
+            "description": "FP64 basis + FP64 MFMA GEMM, derivative matrices stored in FP32, "
+                           "FP64 inverse/log-det/E_L",
+            "value": store["walker_moves_per_s"], "ms_per_step": store["ms_per_step"],
+            "over_fp64": store["walker_moves_per_s"] / fp64["walker_moves_per_s"],
+            "kernels_ms_per_step": {k: v["ms"] / max(args.steps, 1)
+                                    for k, v in store["kernels"].items()}}
+    if args.accuracy and args.precision != "fp64":
+        out["mixed_error"] = accuracy_report(s, args.accuracy, tau=args.tau,
+                                             precision=args.precision)
+        if store is not None:
+            out["fp32store"]["error"] = accuracy_report(s, args.accuracy, tau=args.tau,
+                                                        precision="fp32store")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(s, args.tau, 99)
     print(json.dumps(out))

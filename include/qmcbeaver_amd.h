@@ -37,9 +37,13 @@ typedef enum qmcb_status {
  * Slater matrices, log-determinants, traces, Jastrow and energies are FP64
  * in every mode. */
 typedef enum qmcb_precision {
-  QMCB_FP64 = 0,        /* everything FP64 (reference arithmetic)           */
-  QMCB_MIXED_FP32 = 1,  /* basis + orbital GEMM FP32 (MFMA, FP32 accumulate) */
-  QMCB_MIXED_BF16 = 2   /* basis FP32, GEMM operands BF16, FP32 accumulate  */
+  QMCB_FP64 = 0,        /* everything FP64 (reference arithmetic)                 */
+  QMCB_MIXED_FP32 = 1,  /* orbital GEMM on FP32 MFMA with FP32 operands and
+                           accumulate (basis computed in FP64, rounded); FP32
+                           derivative storage; FP64 re-inversion                  */
+  QMCB_MIXED_BF16 = 2,  /* reserved: GEMM operands BF16, FP32 accumulate         */
+  QMCB_MIXED_F32STORE = 3 /* FP64 basis and GEMM, derivative matrices stored
+                           in FP32 (the HBM-bound part halves)                    */
 } qmcb_precision;
 
 /* Drift (quantum force) modification, QMCSCFJastrow.cpp:329-472. */

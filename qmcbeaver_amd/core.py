@@ -16,7 +16,7 @@ from ._abi import (Options, ResultArrays, Results, SystemBuffers, SystemStruct,
                    VmcStats, PRECISIONS, STATUS, QMCB_DRIFT_NONE)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libqmcb_amd.so")
+LIB_PATH = os.environ.get("QMCB_LIB_PATH") or os.path.join(HERE, "libqmcb_amd.so")
 
 EXPORTS = ["qmcb_abi_version", "qmcb_create", "qmcb_destroy", "qmcb_last_error",
            "qmcb_evaluate", "qmcb_evaluate_device", "qmcb_accept", "qmcb_refresh",

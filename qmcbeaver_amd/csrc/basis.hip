@@ -135,6 +135,8 @@ hipError_t launch_basis(const DevSystem &sys, const PointSet &pts, TS *X, int ld
 
 template hipError_t launch_basis<double, double>(const DevSystem &, const PointSet &, double *,
                                                  int, long, hipStream_t);
+template hipError_t launch_basis<double, float>(const DevSystem &, const PointSet &, float *, int,
+                                                long, hipStream_t);
 template hipError_t launch_basis<float, float>(const DevSystem &, const PointSet &, float *, int,
                                                long, hipStream_t);
 template hipError_t launch_basis<float, __hip_bfloat16>(const DevSystem &, const PointSet &,

@@ -97,7 +97,20 @@ orbital_gemm_kernel(const TI *__restrict__ Ct, int ldc, int n_orb, const TI *__r
   if (!live || p >= P) return;
   const long row = p / o.per_walker;
   const long wk = o.walker_map ? o.walker_map[row] : row;
-  const long base = (long)comp * o.s_comp + wk * o.s_walker + (p % o.per_walker) * o.s_elec;
+  const long pe = (p % o.per_walker) * o.s_elec;
+  if (o.out_val && comp == 0) {
+    double *ov = o.out_val + wk * o.s_walker_val + pe;
+#pragma unroll
+    for (int t = 0; t < MT; t++)
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int orb = o0 + 16 * t + 4 * lk + i;
+        if (orb < n_orb) ov[(long)orb * o.s_orb] = (double)acc[t][i];
+      }
+    return;
+  }
+  const int oc = o.out_val ? comp - 1 : comp;
+  const long base = (long)oc * o.s_comp + wk * o.s_walker + pe;
 #pragma unroll
   for (int t = 0; t < MT; t++) {
 #pragma unroll
@@ -139,5 +152,8 @@ template hipError_t launch_orbital_gemm<double, double>(const double *, int, int
 template hipError_t launch_orbital_gemm<float, float>(const float *, int, int, const float *, int,
                                                       long, long, int, const GemmOut &,
                                                       hipStream_t);
+template hipError_t launch_orbital_gemm<double, float>(const double *, int, int, const double *,
+                                                       int, long, long, int, const GemmOut &,
+                                                       hipStream_t);
 
 }  // namespace qmcb

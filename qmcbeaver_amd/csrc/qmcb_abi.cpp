@@ -49,6 +49,7 @@ struct qmcb_ctx {
   // resident walker state
   double *R = nullptr;
   double *inv[2] = {nullptr, nullptr};
+  double *dval[2] = {nullptr, nullptr};
   void *dmat[2] = {nullptr, nullptr};
   double *logdet[2] = {nullptr, nullptr}, *sgn[2] = {nullptr, nullptr};
   double *gradPR[2] = {nullptr, nullptr}, *lapPR[2] = {nullptr, nullptr};
@@ -101,14 +102,17 @@ namespace {
     }                                                                           \
   } while (0)
 
+// Device allocations are zero-filled on the context's own stream (it is a
+// non-blocking stream: work on the legacy null stream would not be ordered
+// before its kernels); qmcb_create synchronises the stream before returning.
 template <typename T>
 hipError_t dalloc(qmcb_ctx *ctx, T **p, size_t count) {
   void *q = nullptr;
   size_t bytes = std::max<size_t>(count * sizeof(T), 16);
   hipError_t e = hipMalloc(&q, bytes);
   if (e != hipSuccess) return e;
-  e = hipMemset(q, 0, bytes);
   ctx->allocs.push_back(q);
+  e = hipMemsetAsync(q, 0, bytes, ctx->stream);
   *p = (T *)q;
   return e;
 }
@@ -117,9 +121,8 @@ hipError_t dalloc_bytes(qmcb_ctx *ctx, void **p, size_t bytes) {
   bytes = std::max<size_t>(bytes, 16);
   hipError_t e = hipMalloc(p, bytes);
   if (e != hipSuccess) return e;
-  e = hipMemset(*p, 0, bytes);
   ctx->allocs.push_back(*p);
-  return e;
+  return hipMemsetAsync(*p, 0, bytes, ctx->stream);
 }
 
 int pad_orb(int n) {
@@ -195,6 +198,7 @@ SlaterArgs slater_args(qmcb_ctx *ctx, int sp) {
   a.N = ctx->N;
   a.spin_start = ctx->start[sp];
   a.inv = ctx->inv[sp];
+  a.dval = ctx->dval[sp];
   a.dmat = ctx->dmat[sp];
   a.logdet = ctx->logdet[sp];
   a.sgn = ctx->sgn[sp];
@@ -223,7 +227,8 @@ struct Engine {
                      d_map + c0};
         TIMED("basis", (launch_basis<TC, TX>(ctx->dsys, pts, (TX *)ctx->X, ctx->kpad, ctx->ldp,
                                             ctx->stream)));
-        GemmOut go{ctx->dmat[sp], (long)n * n, (long)n, (long)kComp * n * n, 1, n, d_map + c0};
+        GemmOut go{ctx->dmat[sp], (long)n * n, (long)n, (long)kDer * n * n, 1, n, d_map + c0,
+                   ctx->dval[sp], (long)n * n};
         TIMED("orbital_gemm", (launch_orbital_gemm<TX, TS>((const TX *)ctx->Ct[sp], ctx->npad[sp], n,
                                                           (const TX *)ctx->X, ctx->kpad, ctx->ldp,
                                                           (long)cw * n, ctx->kpad, go,
@@ -272,7 +277,8 @@ struct Engine {
     TIMED("basis", (launch_basis<TC, TX>(ctx->dsys, pts, (TX *)ctx->X, ctx->kpad, ctx->ldp,
                                         ctx->stream)));
     const int n = ctx->n[sp];
-    GemmOut go{ctx->vrow, (long)ctx->npad[sp], 1, (long)kComp * ctx->npad[sp], 0, 1, nullptr};
+    GemmOut go{ctx->vrow, (long)ctx->npad[sp], 1, (long)kComp * ctx->npad[sp], 0, 1, nullptr,
+               nullptr, 0};
     TIMED("orbital_gemm", (launch_orbital_gemm<TX, TO>((const TX *)ctx->Ct[sp], ctx->npad[sp], n,
                                                       (const TX *)ctx->X, ctx->kpad, ctx->ldp,
                                                       (long)num, ctx->kpad, go, ctx->stream)));
@@ -333,11 +339,15 @@ struct Engine {
   }
 };
 
+// Engine<TS derivative storage, TX GEMM operand, TC basis arithmetic, TO GEMM output>
 typedef Engine<double, double, double, double> EngineF64;
-typedef Engine<float, float, float, float> EngineF32;
+typedef Engine<float, float, double, float> EngineF32;       // FP64 basis, FP32 MFMA
+typedef Engine<float, double, double, double> EngineF32Store;  // FP64 GEMM, FP32 storage
 
-#define DISPATCH(ctx, call)                                  \
-  ((ctx)->opt.precision == QMCB_FP64 ? EngineF64::call : EngineF32::call)
+#define DISPATCH(ctx, call)                                                          \
+  ((ctx)->opt.precision == QMCB_FP64                                                 \
+       ? EngineF64::call                                                             \
+       : ((ctx)->opt.precision == QMCB_MIXED_F32STORE ? EngineF32Store::call : EngineF32::call))
 
 qmcb_status check_ids(qmcb_ctx *ctx, int num, const int32_t *ids) {
   if (num < 0 || num > ctx->Wc) {
@@ -395,7 +405,8 @@ hipError_t upload(qmcb_ctx *ctx, const T **dst, const T *src, size_t count) {
   T *d = nullptr;
   hipError_t e = dalloc(ctx, &d, count);
   if (e != hipSuccess) return e;
-  if (count) e = hipMemcpy(d, src, count * sizeof(T), hipMemcpyHostToDevice);
+  if (count) e = hipMemcpyAsync(d, src, count * sizeof(T), hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);  // src is pageable host memory
   *dst = d;
   return e;
 }
@@ -415,7 +426,9 @@ qmcb_status create_impl(const qmcb_system *s, const qmcb_options *o, qmcb_ctx *c
   ctx->kpad = ((s->n_basis + 3) / 4) * 4;
   ctx->Wc = o->max_walkers;
   const bool f64 = o->precision == QMCB_FP64;
-  ctx->ts = ctx->tx = ctx->to = f64 ? 8 : 4;
+  const bool store32 = o->precision == QMCB_MIXED_F32STORE;
+  ctx->ts = f64 ? 8 : 4;
+  ctx->tx = ctx->to = (f64 || store32) ? 8 : 4;
   QCHK(hipSetDevice(o->device));
   ctx->device = o->device;
   QCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
@@ -485,12 +498,14 @@ qmcb_status create_impl(const qmcb_system *s, const qmcb_options *o, qmcb_ctx *c
     for (int o2 = 0; o2 < n; o2++)
       for (int b = 0; b < ctx->B; b++) ct[(size_t)b * ctx->npad[sp] + o2] = C[(size_t)o2 * ctx->B + b];
     QCHK(dalloc_bytes(ctx, &ctx->Ct[sp], cnt * ctx->tx));
-    if (f64) {
-      QCHK(hipMemcpy(ctx->Ct[sp], ct.data(), cnt * 8, hipMemcpyHostToDevice));
+    if (ctx->tx == 8) {
+      QCHK(hipMemcpyAsync(ctx->Ct[sp], ct.data(), cnt * 8, hipMemcpyHostToDevice, ctx->stream));
+      QCHK(hipStreamSynchronize(ctx->stream));
     } else {
       std::vector<float> cf(cnt);
       for (size_t i = 0; i < cnt; i++) cf[i] = (float)ct[i];
-      QCHK(hipMemcpy(ctx->Ct[sp], cf.data(), cnt * 4, hipMemcpyHostToDevice));
+      QCHK(hipMemcpyAsync(ctx->Ct[sp], cf.data(), cnt * 4, hipMemcpyHostToDevice, ctx->stream));
+      QCHK(hipStreamSynchronize(ctx->stream));
     }
   }
 
@@ -500,7 +515,8 @@ qmcb_status create_impl(const qmcb_system *s, const qmcb_options *o, qmcb_ctx *c
   for (int sp = 0; sp < 2; sp++) {
     const long n = ctx->n[sp];
     QCHK(dalloc(ctx, &ctx->inv[sp], W * n * n));
-    QCHK(dalloc_bytes(ctx, &ctx->dmat[sp], (size_t)W * kComp * n * n * ctx->ts));
+    QCHK(dalloc(ctx, &ctx->dval[sp], W * n * n));
+    QCHK(dalloc_bytes(ctx, &ctx->dmat[sp], (size_t)W * kDer * n * n * ctx->ts));
     QCHK(dalloc(ctx, &ctx->logdet[sp], W));
     QCHK(dalloc(ctx, &ctx->sgn[sp], W));
     QCHK(dalloc(ctx, &ctx->gradPR[sp], W * std::max(n, 1L) * 3));
@@ -557,6 +573,7 @@ qmcb_status create_impl(const qmcb_system *s, const qmcb_options *o, qmcb_ctx *c
   QCHK(dalloc(ctx, &r.jastrow_u, W));
   QCHK(dalloc(ctx, &r.singular, W));
   ctx->initialized.assign(ctx->Wc, 0);
+  QCHK(hipStreamSynchronize(ctx->stream));
   return QMCB_OK;
 }
 
@@ -570,6 +587,7 @@ qmcb_status ensure_staging(qmcb_ctx *ctx) {
   QCHK(dalloc(ctx, &ctx->st_logdet, W));
   QCHK(dalloc(ctx, &ctx->st_sgn, W));
   QCHK(dalloc(ctx, &ctx->st_singular, W));
+  QCHK(hipStreamSynchronize(ctx->stream));
   return QMCB_OK;
 }
 
@@ -640,7 +658,8 @@ qmcb_status qmcb_create(const qmcb_system *sys, const qmcb_options *opt, qmcb_ct
     g_create_error = "system sizes out of range (1 <= n_dn <= n_up <= 128 electrons per spin)";
     return QMCB_INVALID_ARG;
   }
-  if (opt->precision != QMCB_FP64 && opt->precision != QMCB_MIXED_FP32) {
+  if (opt->precision != QMCB_FP64 && opt->precision != QMCB_MIXED_FP32 &&
+      opt->precision != QMCB_MIXED_F32STORE) {
     g_create_error = "precision not supported by this build";
     return QMCB_INVALID_ARG;
   }
@@ -693,11 +712,12 @@ qmcb_status qmcb_evaluate(qmcb_ctx *ctx, int32_t num, const int32_t *ids, const 
       }
   }
   QCHK(hipMemcpyAsync(ctx->d_ids, ids, sizeof(int) * num, hipMemcpyHostToDevice, ctx->stream));
+  std::vector<double> pr;  // must outlive the asynchronous copy below
   if (whichE == -1) {
     QCHK(hipMemcpyAsync(ctx->Rin, R, sizeof(double) * num * ctx->N * 3, hipMemcpyHostToDevice,
                         ctx->stream));
   } else {
-    std::vector<double> pr((size_t)num * 3);
+    pr.resize((size_t)num * 3);
     for (int i = 0; i < num; i++)
       for (int k = 0; k < 3; k++) pr[3 * i + k] = R[((size_t)i * ctx->N + whichE) * 3 + k];
     QCHK(hipMemcpyAsync(ctx->propR, pr.data(), sizeof(double) * num * 3, hipMemcpyHostToDevice,

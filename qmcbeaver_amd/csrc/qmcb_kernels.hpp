@@ -25,11 +25,15 @@ struct PointSet {
 // point p) is stored at
 //   out + k*s_comp + o*s_orb + (p / per_walker)*s_walker + (p % per_walker)*s_elec
 //   (the walker index goes through walker_map when it is non-null)
+//   With out_val set, component 0 goes to out_val (FP64, walker stride
+//   s_walker_val) and components 1..4 to out at (k-1)*s_comp.
 struct GemmOut {
   void *out;
   long s_comp, s_orb, s_walker, s_elec;
   int per_walker;
   const int *walker_map;
+  double *out_val;
+  long s_walker_val;
 };
 
 // scatter batch rows of positions into resident slots: R[map[b]] = Rin[b]
@@ -53,7 +57,8 @@ struct SlaterArgs {
   int N;                 // all electrons
   int spin_start;        // first electron index of this spin
   double *inv;           // [Wc][n][n]
-  void *dmat;            // [Wc][5][n][n] Tstore
+  double *dval;          // [Wc][n][n] FP64 value matrix, (o, e) = D(e, o)
+  void *dmat;            // [Wc][4][n][n] Tstore: d/dx, d/dy, d/dz, Laplacian
   double *logdet, *sgn;  // [Wc]
   double *gradPR;        // [Wc][n][3]
   double *lapPR;         // [Wc]
@@ -64,6 +69,7 @@ struct SlaterArgs {
   double *ubuf, *zbuf;   // [Wc][kDelay][n]
 };
 constexpr int kDelay = 8;
+constexpr int kDer = 4;  // stored derivative components
 
 template <typename TS>
 hipError_t launch_lu_inverse(const SlaterArgs &a, const int *walkers, int num, hipStream_t st);

@@ -6,9 +6,10 @@
 // 1521-1548, LU in ludcmp/lubksb :1204-1421) and inverseUpdateOneRow
 // (Array2D.h:1566-1610).
 //
-// Storage (per walker, per spin): inv(o,e) row-major, dmat(k,o,e) =
-// d_k D(e,o) (transposed), so the thread that owns electron column e of
-// A^-1 also streams column e of every derivative matrix with coalesced loads.
+// Storage (per walker, per spin): inv(o,e) row-major, dval(o,e) = D(e,o) in
+// FP64, dmat(k,o,e) = d_k D(e,o) for k in {x, y, z, Laplacian} (transposed),
+// so the thread that owns electron column e of A^-1 also streams column e of
+// every derivative matrix with coalesced loads.
 #include "qmcb_kernels.hpp"
 
 namespace qmcb {
@@ -45,7 +46,7 @@ lu_inverse_kernel(SlaterArgs A, const int *walkers, int num) {
   __shared__ int dest[M];
   __shared__ int cur[M];
 
-  const TS *dv = (const TS *)A.dmat + (long)w * kComp * n * n;  // component 0
+  const double *dv = A.dval + (long)w * n * n;  // value matrix, (o, e)
   double a[R][R];
 #pragma unroll
   for (int i = 0; i < R; i++)
@@ -206,7 +207,7 @@ gj_rows_kernel(SlaterArgs A, const int *walkers, int num) {
   __shared__ int ipiv[NP], dest[NP], cur[NP];
   __shared__ double tile[TR][NP + 1];
 
-  const TS *dv = (const TS *)A.dmat + (long)w * kComp * n * n;  // value matrix, (o, e)
+  const double *dv = A.dval + (long)w * n * n;  // value matrix, (o, e)
   double a[H];
 #pragma unroll
   for (int j = 0; j < H; j++) {
@@ -359,17 +360,17 @@ traces_all_kernel(SlaterArgs A, const int *walkers, int num) {
   const int n = A.n;
   __shared__ double part[256];
   const double *inv = A.inv + (long)w * n * n;
-  const TS *dm = (const TS *)A.dmat + (long)w * kComp * n * n;
+  const TS *dm = (const TS *)A.dmat + (long)w * kDer * n * n;
   double lap = 0.0;
   const bool sing = A.singular[w] != 0;
   for (int e = threadIdx.x; e < n; e += blockDim.x) {
     double g0 = 0, g1 = 0, g2 = 0, l = 0;
     for (int o = 0; o < n; o++) {
       const double iv = inv[(long)o * n + e];
-      g0 += ld(dm + 1L * n * n + (long)o * n + e) * iv;
-      g1 += ld(dm + 2L * n * n + (long)o * n + e) * iv;
-      g2 += ld(dm + 3L * n * n + (long)o * n + e) * iv;
-      l += ld(dm + 4L * n * n + (long)o * n + e) * iv;
+      g0 += ld(dm + (long)o * n + e) * iv;
+      g1 += ld(dm + 1L * n * n + (long)o * n + e) * iv;
+      g2 += ld(dm + 2L * n * n + (long)o * n + e) * iv;
+      l += ld(dm + 3L * n * n + (long)o * n + e) * iv;
     }
     if (!sing) {
       double *g = A.gradPR + ((long)w * n + e) * 3;
@@ -440,8 +441,28 @@ __device__ inline double drift_factor_unequal(const double g[3], const double x[
 // Pass 1 streams A_0 and the four derivative matrices once, forms the current
 // inverse element on the fly and accumulates w, T1, T2 per column (9 registers).
 // Thread mapping: c = t % NC (electron column), sl = t / NC (row slice).
+#ifndef QMCB_SM_UNROLL
+#define QMCB_SM_UNROLL 4
+#endif
+#ifndef QMCB_SM_MINWAVES
+#define QMCB_SM_MINWAVES 1
+#endif
+#ifndef QMCB_SM_NT
+#define QMCB_SM_NT 1
+#endif
+
+// streamed-once operands: optionally non-temporal loads
+template <typename T>
+__device__ __forceinline__ T ld_stream(const T *p) {
+#if QMCB_SM_NT
+  return __builtin_nontemporal_load(p);
+#else
+  return *p;
+#endif
+}
+
 template <typename TS, typename TO, int NC, int NT>
-__global__ void __launch_bounds__(NT)
+__global__ void __launch_bounds__(NT, QMCB_SM_MINWAVES)
 sm_move_kernel(MoveArgs A, const int *walkers, int num) {
   constexpr int SL = NT / NC;
   constexpr int NW = NT / 64;
@@ -493,17 +514,18 @@ sm_move_kernel(MoveArgs A, const int *walkers, int num) {
   __syncthreads();
 
   // ---- pass 1
-  const TS *dm = (const TS *)A.s.dmat + (long)w * kComp * nn2;
+  const TS *dm = (const TS *)A.s.dmat + (long)w * kDer * nn2;
   double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // w, T1[4], T2[4]
   if (c < n) {
     double *ip = inv + c;
-    const TS *d1 = dm + 1 * nn2 + c, *d2 = dm + 2 * nn2 + c, *d3 = dm + 3 * nn2 + c,
-             *d4 = dm + 4 * nn2 + c;
-#pragma unroll 4
+    const TS *d1 = dm + c, *d2 = dm + 1 * nn2 + c, *d3 = dm + 2 * nn2 + c,
+             *d4 = dm + 3 * nn2 + c;
+#pragma unroll QMCB_SM_UNROLL
     for (int o = o_beg; o < o_end; o++) {
       const long off = (long)o * n;
-      double a = ip[off];
-      const double x = ld(d1 + off), y = ld(d2 + off), z = ld(d3 + off), l = ld(d4 + off);
+      double a = ld_stream(ip + off);
+      const double x = (double)ld_stream(d1 + off), y = (double)ld_stream(d2 + off),
+                   z = (double)ld_stream(d3 + off), l = (double)ld_stream(d4 + off);
 #pragma unroll
       for (int q = 0; q < KD; q++) a -= ub[q][o] * zc[q];
       if (flush) ip[off] = a;
@@ -706,10 +728,12 @@ sm_move_kernel(MoveArgs A, const int *walkers, int num) {
       double *g = A.s.gradPR + ((long)w * n + c) * 3;
       g[0] = gq[0][c]; g[1] = gq[1][c]; g[2] = gq[2][c];
     }
-    TS *dmw = (TS *)A.s.dmat + (long)w * kComp * nn2;
+    TS *dmw = (TS *)A.s.dmat + (long)w * kDer * nn2;
+    double *dvw = A.s.dval + (long)w * nn2;
     for (int i = t; i < kComp * n; i += NT) {
       const int k = i / n, o = i % n;
-      dmw[(long)k * nn2 + (long)o * n + r] = (TS)v[k][o];
+      if (k == 0) dvw[(long)o * n + r] = v[0][o];
+      else dmw[(long)(k - 1) * nn2 + (long)o * n + r] = (TS)v[k][o];
     }
     double *Uxw = A.Ux + (long)w * N * 3;
     for (int i = t; i < 3 * N; i += NT) Uxw[i] = pUx[i];
@@ -779,11 +803,13 @@ commit_kernel(MoveArgs A, const int *walkers, const uint8_t *mask, int num) {
   const double *sg = A.st_grad + (long)b * n * 3;
   for (int i = t; i < 3 * n; i += blockDim.x) g[i] = sg[i];
   const TO *vr = (const TO *)A.vrow;
-  TS *dmw = (TS *)A.s.dmat + (long)w * kComp * nn2;
+  TS *dmw = (TS *)A.s.dmat + (long)w * kDer * nn2;
+  double *dvw = A.s.dval + (long)w * nn2;
   for (int i = t; i < kComp * n; i += blockDim.x) {
     const int k = i / n, o = i % n;
-    dmw[(long)k * nn2 + (long)o * n + r] =
-        (TS)(double)vr[(long)b * A.v_s_walker + (long)k * A.v_s_comp + o];
+    const double x = (double)vr[(long)b * A.v_s_walker + (long)k * A.v_s_comp + o];
+    if (k == 0) dvw[(long)o * n + r] = x;
+    else dmw[(long)(k - 1) * nn2 + (long)o * n + r] = (TS)x;
   }
   double *Uxw = A.Ux + (long)w * N * 3;
   const double *pUx = A.pUx + (long)b * N * 3;
@@ -816,6 +842,7 @@ hipError_t launch_commit(const MoveArgs &a, const int *walkers, const uint8_t *m
                                             hipStream_t);
 INST(double, double)
 INST(float, float)
+INST(float, double)
 #undef INST
 template hipError_t launch_lu_inverse<double>(const SlaterArgs &, const int *, int, hipStream_t);
 template hipError_t launch_lu_inverse<float>(const SlaterArgs &, const int *, int, hipStream_t);
