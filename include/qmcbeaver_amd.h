@@ -192,6 +192,46 @@ typedef struct qmcb_vmc_stats {
 qmcb_status qmcb_vmc_steps(qmcb_ctx *ctx, int32_t num, int64_t first_step,
                            int32_t n_steps, double tau, qmcb_vmc_stats *stats);
 
+/* ---- device-resident DMC ------------------------------------------------
+ * Fixed-node importance-sampled DMC with single-electron moves: Metropolis
+ * with node-crossing rejection (QMCWalker.cpp:1578-1654), the
+ * umrigar93_probability_weighted reweighting once per electron sweep
+ * (QMCWalker.cpp:1060-1185), non-unit-weight branching after each sweep
+ * (QMCRun.cpp:740-814, 1179-1206) and the trial-energy feedback
+ * E_T = E_est - pop_control * ln(W_total / target)
+ * (QMCManager.cpp:2263-2310).  Walker states are copied on the device when a
+ * walker splits; the population lives in the context's slots. */
+typedef struct qmcb_dmc_params {
+  double tau;               /* time step                                       */
+  double e_trial;           /* initial trial energy (used when first_step == 0
+                               or no estimate exists yet)                      */
+  double pop_control;       /* population_control_parameter (reference 1.0)   */
+  double branch_threshold;  /* split walkers above this weight (2.0)          */
+  double fusion_threshold;  /* fuse walkers below this weight (0.45)          */
+  int32_t target_walkers;   /* population the E_T feedback aims at            */
+  int32_t reweight_after;   /* weights stay 1 before this step (100)          */
+} qmcb_dmc_params;
+
+typedef struct qmcb_dmc_stats {
+  double sum_w_e_local;     /* sum over steps and walkers of w E_L             */
+  double sum_w_e_local_sq;
+  double sum_weights;
+  double sum_accept;
+  int64_t n_moves;
+  int32_t population;       /* walkers alive after the last step               */
+  int32_t n_split, n_killed, n_fused;
+  double total_weight;
+  double e_trial;           /* trial energy after the last update              */
+} qmcb_dmc_stats;
+
+/* Slots [0, num) (all-electron evaluated) become the DMC population, weight 1. */
+qmcb_status qmcb_dmc_init(qmcb_ctx *ctx, int32_t num);
+qmcb_status qmcb_dmc_steps(qmcb_ctx *ctx, int64_t first_step, int32_t n_steps,
+                           const qmcb_dmc_params *params, qmcb_dmc_stats *stats);
+/* Current population: up to cap slot ids and weights; *count = population. */
+qmcb_status qmcb_dmc_population(qmcb_ctx *ctx, int32_t cap, int32_t *slots,
+                                double *weights, int32_t *count);
+
 qmcb_status qmcb_sync(qmcb_ctx *ctx);
 
 /* Per-kernel timing of the last qmcb_vmc_steps call, measured with HIP

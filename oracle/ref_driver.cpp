@@ -16,12 +16,16 @@
 // configs.bin: int32 nconf, int32 N, double R[nconf][N][3]
 // moves.bin:   int32 nwalk, int32 N, int32 nmoves, then per walker
 //              double R[N][3] followed by nmoves x double (e, x, y, z)
-// out.bin:     per evaluation one record of 11 + 6N doubles:
-//              detA detB U E_L KE PE V_ne V_ee singular modRatio lnPsi,
+// out.bin:     per evaluation one record of 12 + 6N doubles:
+//              detA detB U E_L KE PE V_ne V_ee singular modRatio lnPsi signPsi,
 //              gradPsiRatio[N][3], modifiedGradPsiRatio[N][3]
+//              (detA/detB only with one_e_per_iter 1; lnPsi and its sign come
+//              from the QMCDouble psi = k a^b e^c as ln|k| + b ln a + c)
 #include <cstdio>
 #include <cstdlib>
 #include <cmath>
+#include <sstream>
+#include <string>
 #include <vector>
 #include <sys/time.h>
 
@@ -31,11 +35,30 @@
 
 QMCInput globalInput;
 
+// ln|psi| and sign from the printed k ( a ^ b ) Exp[ c ] form of a QMCDouble
+static void log_qmcdouble(const QMCDouble &q, double &lnv, double &sgn) {
+  std::ostringstream os;
+  os.precision(17);
+  os << q;
+  std::string t = os.str();
+  for (size_t i = 0; i < t.size(); i++)
+    if (t[i] == '(' || t[i] == ')' || t[i] == '^' || t[i] == '[' || t[i] == ']') t[i] = ' ';
+  std::istringstream is(t);
+  double k = 0, a = 1, b = 0, c = 0;
+  std::string ex;
+  is >> k >> a >> b >> ex >> c;
+  lnv = std::log(std::fabs(k)) + (b != 0.0 ? b * std::log(a) : 0.0) + c;
+  sgn = k < 0 ? -1.0 : 1.0;
+}
+
 static void record(std::FILE *f, QMCWalkerData &wd, int N) {
   int na = globalInput.WF.getNumberElectrons(true);
   int nb = globalInput.WF.getNumberElectrons(false);
-  double detA = na > 0 ? wd.DcA(0) : 1.0;
-  double detB = nb > 0 ? wd.DcB(0) : 1.0;
+  const bool onee = globalInput.flags.one_e_per_iter != 0;
+  double detA = (onee && na > 0) ? wd.DcA(0) : 1.0;
+  double detB = (onee && nb > 0) ? wd.DcB(0) : 1.0;
+  double lnpsi, sgn;
+  log_qmcdouble(wd.psi, lnpsi, sgn);
   std::vector<double> r;
   r.push_back(detA);
   r.push_back(detB);
@@ -47,7 +70,8 @@ static void record(std::FILE *f, QMCWalkerData &wd, int N) {
   r.push_back(wd.eeEnergy);
   r.push_back(wd.singular ? 1.0 : 0.0);
   r.push_back(wd.modificationRatio);
-  r.push_back(std::log(std::fabs(detA)) + std::log(std::fabs(detB)) + wd.U);
+  r.push_back(lnpsi);
+  r.push_back(sgn);
   for (int e = 0; e < N; e++)
     for (int k = 0; k < 3; k++) r.push_back(wd.gradPsiRatio(e, k));
   for (int e = 0; e < N; e++)

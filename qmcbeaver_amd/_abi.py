@@ -57,6 +57,20 @@ class VmcStats(C.Structure):
                 ("n_samples", C.c_int64)]
 
 
+class DmcParams(C.Structure):
+    _fields_ = [("tau", C.c_double), ("e_trial", C.c_double), ("pop_control", C.c_double),
+                ("branch_threshold", C.c_double), ("fusion_threshold", C.c_double),
+                ("target_walkers", C.c_int32), ("reweight_after", C.c_int32)]
+
+
+class DmcStats(C.Structure):
+    _fields_ = [("sum_w_e_local", C.c_double), ("sum_w_e_local_sq", C.c_double),
+                ("sum_weights", C.c_double), ("sum_accept", C.c_double),
+                ("n_moves", C.c_int64), ("population", C.c_int32), ("n_split", C.c_int32),
+                ("n_killed", C.c_int32), ("n_fused", C.c_int32), ("total_weight", C.c_double),
+                ("e_trial", C.c_double)]
+
+
 def _pade_struct(row) -> Pade:
     p = Pade()
     p.n_num, p.n_den = int(row[0]), int(row[1])

@@ -13,7 +13,7 @@ from typing import Dict, Optional, Sequence
 import numpy as np
 
 from ._abi import (Options, ResultArrays, Results, SystemBuffers, SystemStruct,
-                   VmcStats, PRECISIONS, STATUS, QMCB_DRIFT_NONE)
+                   VmcStats, DmcParams, DmcStats, PRECISIONS, STATUS, QMCB_DRIFT_NONE)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("QMCB_LIB_PATH") or os.path.join(HERE, "libqmcb_amd.so")
@@ -21,7 +21,8 @@ LIB_PATH = os.environ.get("QMCB_LIB_PATH") or os.path.join(HERE, "libqmcb_amd.so
 EXPORTS = ["qmcb_abi_version", "qmcb_create", "qmcb_destroy", "qmcb_last_error",
            "qmcb_evaluate", "qmcb_evaluate_device", "qmcb_accept", "qmcb_refresh",
            "qmcb_get_positions", "qmcb_vmc_steps", "qmcb_sync", "qmcb_kernel_times",
-           "qmcb_enable_kernel_timing"]
+           "qmcb_enable_kernel_timing", "qmcb_dmc_init", "qmcb_dmc_steps",
+           "qmcb_dmc_population"]
 
 _lib = None
 
@@ -67,6 +68,13 @@ def lib():
     L.qmcb_kernel_times.argtypes = [vp, C.c_int32, C.POINTER(C.c_char_p), dp,
                                     C.POINTER(C.c_int64)]
     L.qmcb_enable_kernel_timing.argtypes = [vp, C.c_int32]
+    L.qmcb_dmc_init.restype = C.c_int
+    L.qmcb_dmc_init.argtypes = [vp, C.c_int32]
+    L.qmcb_dmc_steps.restype = C.c_int
+    L.qmcb_dmc_steps.argtypes = [vp, C.c_int64, C.c_int32, C.POINTER(DmcParams),
+                                 C.POINTER(DmcStats)]
+    L.qmcb_dmc_population.restype = C.c_int
+    L.qmcb_dmc_population.argtypes = [vp, C.c_int32, ip, dp, C.POINTER(C.c_int32)]
     _lib = L
     return L
 
@@ -161,6 +169,28 @@ class Context:
         self._check(lib().qmcb_vmc_steps(self._h, num, first_step, n_steps, tau,
                                          C.byref(st)), "qmcb_vmc_steps")
         return st
+
+    def dmc_init(self, num: int):
+        self._check(lib().qmcb_dmc_init(self._h, num), "qmcb_dmc_init")
+
+    def dmc_steps(self, first_step: int, n_steps: int, tau: float, e_trial: float, *,
+                  target: int = 0, pop_control: float = 1.0, branch_threshold: float = 2.0,
+                  fusion_threshold: float = 0.45, reweight_after: int = 100) -> DmcStats:
+        p = DmcParams(tau, e_trial, pop_control, branch_threshold, fusion_threshold, target,
+                      reweight_after)
+        st = DmcStats()
+        self._check(lib().qmcb_dmc_steps(self._h, first_step, n_steps, C.byref(p),
+                                         C.byref(st)), "qmcb_dmc_steps")
+        return st
+
+    def dmc_population(self):
+        cap = self.max_walkers
+        slots = np.zeros(cap, dtype=np.int32)
+        w = np.zeros(cap)
+        n = C.c_int32()
+        self._check(lib().qmcb_dmc_population(self._h, cap, _ip(slots), _dp(w), C.byref(n)),
+                    "qmcb_dmc_population")
+        return slots[:n.value].copy(), w[:n.value].copy()
 
     def sync(self):
         self._check(lib().qmcb_sync(self._h), "qmcb_sync")

@@ -12,6 +12,7 @@
 #include <map>
 #include <memory>
 #include <new>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -78,6 +79,13 @@ struct qmcb_ctx {
   uint8_t *d_mask = nullptr;
   // vmc accumulators
   double *acc_el = nullptr, *acc_el2 = nullptr, *acc_n = nullptr, *acc_prop_el = nullptr;
+  // dmc population
+  double *weight = nullptr, *acc_w = nullptr;
+  int *d_active = nullptr, *d_pairs = nullptr;
+  std::vector<int> active, free_slots;
+  bool dmc_ready = false;
+  bool have_e_trial = false;
+  double e_trial = 0.0;
   // staged single-electron pass
   int staged_moved = -1;
   int staged_num = 0;
@@ -314,6 +322,8 @@ struct Engine {
     a.unif = ctx->unif;
     a.acc_el = ctx->acc_el; a.acc_el2 = ctx->acc_el2; a.acc_n = ctx->acc_n;
     a.acc_prop_el = ctx->acc_prop_el; a.cur_el = ctx->cur_el;
+    a.weight = ctx->weight;
+    a.acc_w = ctx->acc_w;
     return a;
   }
 
@@ -548,6 +558,10 @@ qmcb_status create_impl(const qmcb_system *s, const qmcb_options *o, qmcb_ctx *c
   QCHK(dalloc(ctx, &ctx->acc_el2, W));
   QCHK(dalloc(ctx, &ctx->acc_n, W));
   QCHK(dalloc(ctx, &ctx->acc_prop_el, W));
+  QCHK(dalloc(ctx, &ctx->weight, W));
+  QCHK(dalloc(ctx, &ctx->acc_w, W));
+  QCHK(dalloc(ctx, &ctx->d_active, W));
+  QCHK(dalloc(ctx, &ctx->d_pairs, 2 * W));
   QCHK(dalloc(ctx, &ctx->d_ids, W));
   QCHK(dalloc(ctx, &ctx->d_mask, W));
   // basis scratch: at least one point per walker; all-electron chunks reuse it
@@ -873,6 +887,248 @@ qmcb_status qmcb_vmc_steps(qmcb_ctx *ctx, int32_t num, int64_t first_step, int32
     stats->n_samples = (int64_t)num * n_steps;
   }
   tm_collect(ctx);
+  return QMCB_OK;
+}
+
+static CopyArgs walker_copy_args(qmcb_ctx *ctx) {
+  CopyArgs c{};
+  auto add = [&](void *p, long bytes) {
+    if (bytes <= 0) return;
+    c.base[c.narr] = (char *)p;
+    c.bytes[c.narr] = bytes;
+    c.narr++;
+  };
+  const long N = ctx->N;
+  add(ctx->R, N * 3 * 8);
+  for (int sp = 0; sp < 2; sp++) {
+    const long n = ctx->n[sp];
+    if (n == 0) continue;
+    add(ctx->inv[sp], n * n * 8);
+    add(ctx->dval[sp], n * n * 8);
+    add(ctx->dmat[sp], kDer * n * n * (long)ctx->ts);
+    add(ctx->ubuf[sp], kDelay * n * 8);
+    add(ctx->zbuf[sp], kDelay * n * 8);
+    add(ctx->dcount[sp], 4);
+    add(ctx->logdet[sp], 8);
+    add(ctx->sgn[sp], 8);
+    add(ctx->gradPR[sp], n * 3 * 8);
+    add(ctx->lapPR[sp], 8);
+    add(ctx->singular[sp], 4);
+  }
+  add(ctx->U, 8);
+  add(ctx->Ux, N * 3 * 8);
+  add(ctx->Uxx, 8);
+  add(ctx->Vee, 8);
+  add(ctx->Ven, 8);
+  add(ctx->cur_el, 8);
+  add(ctx->weight, 8);
+  return c;
+}
+
+qmcb_status qmcb_dmc_init(qmcb_ctx *ctx, int32_t num) {
+  if (!ctx || num < 1 || num > ctx->Wc) return QMCB_INVALID_ARG;
+  for (int i = 0; i < num; i++)
+    if (!ctx->initialized[i]) {
+      ctx->err = "dmc slots must be initialised by an all-electron evaluate";
+      return QMCB_BAD_STATE;
+    }
+  QCHK(hipSetDevice(ctx->device));
+  ctx->active.resize(num);
+  for (int i = 0; i < num; i++) ctx->active[i] = i;
+  ctx->free_slots.clear();
+  for (int i = ctx->Wc - 1; i >= num; i--) ctx->free_slots.push_back(i);
+  std::vector<double> ones(ctx->Wc, 1.0);
+  QCHK(hipMemcpyAsync(ctx->weight, ones.data(), sizeof(double) * ctx->Wc, hipMemcpyHostToDevice,
+                      ctx->stream));
+  QCHK(hipMemcpyAsync(ctx->d_active, ctx->active.data(), sizeof(int) * num,
+                      hipMemcpyHostToDevice, ctx->stream));
+  QCHK(hipStreamSynchronize(ctx->stream));
+  ctx->dmc_ready = true;
+  ctx->have_e_trial = false;
+  return QMCB_OK;
+}
+
+qmcb_status qmcb_dmc_steps(qmcb_ctx *ctx, int64_t first_step, int32_t n_steps,
+                           const qmcb_dmc_params *prm, qmcb_dmc_stats *stats) {
+  if (!ctx || !prm || n_steps < 0 || !(prm->tau > 0)) return QMCB_INVALID_ARG;
+  if (!ctx->dmc_ready) {
+    ctx->err = "qmcb_dmc_init first";
+    return QMCB_BAD_STATE;
+  }
+  if (ctx->opt.drift_mod != QMCB_DRIFT_NONE) {
+    ctx->err = "device DMC supports QF_modification_type none";
+    return QMCB_INVALID_ARG;
+  }
+  QCHK(hipSetDevice(ctx->device));
+  const int W = ctx->Wc;
+  QCHK(hipMemsetAsync(ctx->acc_el, 0, sizeof(double) * W, ctx->stream));
+  QCHK(hipMemsetAsync(ctx->acc_el2, 0, sizeof(double) * W, ctx->stream));
+  QCHK(hipMemsetAsync(ctx->acc_n, 0, sizeof(double) * W, ctx->stream));
+  QCHK(hipMemsetAsync(ctx->acc_prop_el, 0, sizeof(double) * W, ctx->stream));
+  QCHK(hipMemsetAsync(ctx->acc_w, 0, sizeof(double) * W, ctx->stream));
+  if (!ctx->have_e_trial) {
+    ctx->e_trial = prm->e_trial;
+    ctx->have_e_trial = true;
+  }
+  const int target = prm->target_walkers > 0 ? prm->target_walkers : (int)ctx->active.size();
+  std::mt19937_64 rng(ctx->opt.seed ^ (0x9E3779B97F4A7C15ULL * (uint64_t)(first_step + 1)));
+  std::uniform_real_distribution<double> unif(0.0, 1.0);
+  qmcb_dmc_stats st{};
+  std::vector<double> wts(W), acc_e(W), acc_w(W);
+  const CopyArgs cargs = walker_copy_args(ctx);
+  long moves = 0;
+  for (int s = 0; s < n_steps; s++) {
+    const long step = first_step + s;
+    const int m = (int)(step % ctx->N);
+    const int pop = (int)ctx->active.size();
+    if (pop == 0) {
+      ctx->err = "DMC population died out";
+      return QMCB_BAD_STATE;
+    }
+    ProposeArgs pa{};
+    pa.N = ctx->N;
+    pa.moved = m;
+    pa.n_up = ctx->n[0];
+    pa.R = ctx->R;
+    pa.gradPR[0] = ctx->gradPR[0];
+    pa.gradPR[1] = ctx->gradPR[1];
+    pa.Ux = ctx->Ux;
+    pa.tau = prm->tau;
+    pa.seed = ctx->opt.seed;
+    pa.step = step;
+    pa.propR = ctx->propR;
+    pa.gauss = ctx->gauss;
+    pa.unif = ctx->unif;
+    TIMED("propose", launch_propose(pa, ctx->d_active, pop, ctx->stream));
+    qmcb_status sst = DISPATCH(ctx, move_front(ctx, ctx->d_active, pop, m));
+    if (sst != QMCB_OK) return sst;
+    qmcb_results none{};
+    MoveArgs a = DISPATCH(ctx, move_args(ctx, m, 2, none));
+    a.tau = prm->tau;
+    a.e_trial = ctx->e_trial;
+    a.tau_eff = prm->tau;
+    a.reweight = (m == 0 && step >= prm->reweight_after) ? 1 : 0;
+    sst = DISPATCH(ctx, sm(ctx, a, ctx->d_active, pop));
+    if (sst != QMCB_OK) return sst;
+    moves += pop;
+    const int sp = m < ctx->n[0] ? 0 : 1;
+    if (ctx->opt.refresh_interval > 0 && ++ctx->sm_updates[sp] >= ctx->opt.refresh_interval) {
+      ctx->sm_updates[sp] = 0;
+      sst = DISPATCH(ctx, refresh(ctx, ctx->d_active, pop, 1 << sp));
+      if (sst != QMCB_OK) return sst;
+    }
+    if (m != 0) continue;
+    // ---- branching once per electron sweep (after the electron-0 move)
+    QCHK(hipMemcpyAsync(wts.data(), ctx->weight, sizeof(double) * W, hipMemcpyDeviceToHost,
+                        ctx->stream));
+    QCHK(hipMemcpyAsync(acc_e.data(), ctx->acc_el, sizeof(double) * W, hipMemcpyDeviceToHost,
+                        ctx->stream));
+    QCHK(hipMemcpyAsync(acc_w.data(), ctx->acc_w, sizeof(double) * W, hipMemcpyDeviceToHost,
+                        ctx->stream));
+    QCHK(hipStreamSynchronize(ctx->stream));
+    std::vector<int> next, pairs, dead;
+    next.reserve(pop * 2);
+    int fuse_pending = -1;
+    for (int idx = 0; idx < pop; idx++) {
+      const int w = ctx->active[idx];
+      const double wt = wts[w];
+      if (wt > prm->branch_threshold && !ctx->free_slots.empty()) {
+        wts[w] = wt / 2.0;
+        const int d = ctx->free_slots.back();
+        ctx->free_slots.pop_back();
+        pairs.push_back(w);
+        pairs.push_back(d);
+        wts[d] = wt / 2.0;
+        next.push_back(w);
+        next.push_back(d);
+        st.n_split++;
+      } else if (wt < 0.01) {
+        dead.push_back(w);
+        st.n_killed++;
+      } else if (wt < prm->fusion_threshold) {
+        if (fuse_pending < 0) {
+          fuse_pending = w;
+        } else {
+          const double w1 = wts[fuse_pending], w3 = w1 + wt;
+          if (unif(rng) < w1 / w3) {
+            wts[fuse_pending] = w3;
+            next.push_back(fuse_pending);
+            dead.push_back(w);
+          } else {
+            wts[w] = w3;
+            next.push_back(w);
+            dead.push_back(fuse_pending);
+          }
+          fuse_pending = -1;
+          st.n_fused++;
+        }
+      } else {
+        next.push_back(w);
+      }
+    }
+    if (fuse_pending >= 0) next.push_back(fuse_pending);
+    for (int w : dead) ctx->free_slots.push_back(w);
+    ctx->active.swap(next);
+    QCHK(hipMemcpyAsync(ctx->weight, wts.data(), sizeof(double) * W, hipMemcpyHostToDevice,
+                        ctx->stream));
+    if (!pairs.empty()) {
+      QCHK(hipMemcpyAsync(ctx->d_pairs, pairs.data(), sizeof(int) * pairs.size(),
+                          hipMemcpyHostToDevice, ctx->stream));
+      TIMED("copy_walkers", launch_copy_walkers(cargs, ctx->d_pairs, (int)pairs.size() / 2,
+                                                ctx->stream));
+    }
+    QCHK(hipMemcpyAsync(ctx->d_active, ctx->active.data(), sizeof(int) * ctx->active.size(),
+                        hipMemcpyHostToDevice, ctx->stream));
+    QCHK(hipStreamSynchronize(ctx->stream));  // host vectors above must outlive the copies
+    // trial energy: E_T = E_est - pop_control * ln(W_total / target)
+    double se = 0.0, sw = 0.0, wtot = 0.0;
+    for (int i = 0; i < W; i++) { se += acc_e[i]; sw += acc_w[i]; }
+    for (int w : ctx->active) wtot += wts[w];
+    if (sw > 0 && step >= prm->reweight_after) {
+      const double ratio = wtot / target;
+      const double lr = ratio > 1e-250 ? std::log(ratio) : 0.0;
+      ctx->e_trial = se / sw - prm->pop_control * lr;
+    }
+  }
+  // statistics
+  {
+    std::vector<double> a(W), b2(W), c(W), d(W), wv(W);
+    QCHK(hipMemcpyAsync(a.data(), ctx->acc_el, sizeof(double) * W, hipMemcpyDeviceToHost, ctx->stream));
+    QCHK(hipMemcpyAsync(b2.data(), ctx->acc_el2, sizeof(double) * W, hipMemcpyDeviceToHost, ctx->stream));
+    QCHK(hipMemcpyAsync(c.data(), ctx->acc_n, sizeof(double) * W, hipMemcpyDeviceToHost, ctx->stream));
+    QCHK(hipMemcpyAsync(d.data(), ctx->acc_w, sizeof(double) * W, hipMemcpyDeviceToHost, ctx->stream));
+    QCHK(hipMemcpyAsync(wv.data(), ctx->weight, sizeof(double) * W, hipMemcpyDeviceToHost, ctx->stream));
+    QCHK(hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < W; i++) {
+      st.sum_w_e_local += a[i];
+      st.sum_w_e_local_sq += b2[i];
+      st.sum_accept += c[i];
+      st.sum_weights += d[i];
+    }
+    for (int w : ctx->active) st.total_weight += wv[w];
+    st.n_moves = moves;
+    st.population = (int)ctx->active.size();
+    st.e_trial = ctx->e_trial;
+  }
+  if (stats) *stats = st;
+  tm_collect(ctx);
+  return QMCB_OK;
+}
+
+qmcb_status qmcb_dmc_population(qmcb_ctx *ctx, int32_t cap, int32_t *slots, double *weights,
+                                int32_t *count) {
+  if (!ctx || cap < 0) return QMCB_INVALID_ARG;
+  QCHK(hipSetDevice(ctx->device));
+  const int pop = (int)ctx->active.size();
+  if (count) *count = pop;
+  std::vector<double> wv(ctx->Wc);
+  QCHK(hipMemcpyAsync(wv.data(), ctx->weight, sizeof(double) * ctx->Wc, hipMemcpyDeviceToHost,
+                      ctx->stream));
+  QCHK(hipStreamSynchronize(ctx->stream));
+  for (int i = 0; i < pop && i < cap; i++) {
+    if (slots) slots[i] = ctx->active[i];
+    if (weights) weights[i] = wv[ctx->active[i]];
+  }
   return QMCB_OK;
 }
 

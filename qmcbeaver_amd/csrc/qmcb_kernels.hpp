@@ -105,8 +105,22 @@ struct MoveArgs {
   const double *gauss;   // [Wc][3] the Gaussian displacement used (vmc)
   const double *unif;    // [Wc] uniform for the accept test (vmc)
   double *acc_el, *acc_el2, *acc_n, *acc_prop_el, *cur_el;  // [Wc]
-  void *dmat_refresh_dummy;
+  // diffusion Monte Carlo (mode 2): fixed-node rejection, weights
+  double *weight;        // [Wc]
+  double *acc_w;         // [Wc] sum of weights over the steps
+  double e_trial, tau_eff;
+  int reweight;          // reweight this step (electron 0 moved)
 };
+
+// copy whole walker states slot -> slot (DMC branching)
+constexpr int kMaxCopyArrays = 40;
+struct CopyArgs {
+  int narr;
+  char *base[kMaxCopyArrays];
+  long bytes[kMaxCopyArrays];  // bytes per walker (multiple of 4)
+};
+hipError_t launch_copy_walkers(const CopyArgs &a, const int *pairs /*[2*np]*/, int np,
+                               hipStream_t st);
 
 template <typename TS, typename TO>
 hipError_t launch_sm_move(const MoveArgs &a, const int *walkers, int num, hipStream_t st);

@@ -476,7 +476,7 @@ sm_move_kernel(MoveArgs A, const int *walkers, int num) {
   const int rows = (n + SL - 1) / SL;
   const int o_beg = sl * rows, o_end = min(n, o_beg + rows);
   const int dc = A.s.dcount[w];
-  const bool flush = (A.mode == 1) && dc >= KD;
+  const bool flush = (A.mode >= 1) && dc >= KD;
 
   __shared__ double v[kComp][NC];
   __shared__ double colr[NC];
@@ -603,7 +603,7 @@ sm_move_kernel(MoveArgs A, const int *walkers, int num) {
       double *o = A.out.grad_lnpsi + ((long)b * N + e) * 3;
       o[0] = g[0]; o[1] = g[1]; o[2] = g[2];
     }
-    if (A.mode == 1) {
+    if (A.mode >= 1) {
       // Green's function terms (QMCWalker.cpp:343-398, 756-780); drift =
       // gradPsiRatio (QF_modification_type none)
       if (e != A.moved) {
@@ -708,14 +708,17 @@ sm_move_kernel(MoveArgs A, const int *walkers, int num) {
     return;
   }
 
-  // ---- mode 1: Metropolis with the Green's function ratio
-  // (QMCWalker.cpp:1578-1685): p = |Psi'/Psi|^2 G(R<-R')/G(R'<-R)
+  // ---- modes 1 (VMC) and 2 (DMC): Metropolis with the Green's function
+  // ratio (QMCWalker.cpp:1578-1685): p = |Psi'/Psi|^2 G(R<-R')/G(R'<-R);
+  // DMC additionally rejects node crossings (fixed node, :1641-1646)
   const double logpsi_old = ld_s + ld_o + A.U[w];
   double p = sing ? 0.0
                   : exp(2.0 * (logpsi_new - logpsi_old) - (s_bc[2] - s_bc[1]) / (2.0 * tau));
   if (!(p == p)) p = 0.0;
+  if (A.mode == 2 && ratio < 0) p = 0.0;
   if (p > 1.0) p = 1.0;
   const bool accept = p > A.unif[b];
+  const double el_old = A.cur_el[w];
   __syncthreads();  // every read of the current state is done
   const int base = flush ? 0 : dc;
   if (accept) {
@@ -754,12 +757,47 @@ sm_move_kernel(MoveArgs A, const int *walkers, int num) {
     A.s.dcount[w] = 0;
   }
   if (t == 0) {
-    const double el = accept ? EL : A.cur_el[w];
-    A.acc_el[w] += el;
-    A.acc_el2[w] += el * el;
+    const double el = accept ? EL : el_old;
+    double wt = 1.0;
+    if (A.mode == 2) {
+      // umrigar93_probability_weighted reweighting once per electron sweep
+      // (QMCWalker.cpp:1060-1185, UNR93 eq. 25): dW = exp(tau_eff *
+      // (p (S'+S)/2 + (1-p) S)), S = E_T - E_L
+      wt = A.weight[w];
+      if (A.reweight) {
+        const double so_ = A.e_trial - el_old;
+        const double st_ = (p > 0.0) ? A.e_trial - EL : 0.0;
+        wt *= exp(A.tau_eff * (p * 0.5 * (so_ + st_) + (1.0 - p) * so_));
+        A.weight[w] = wt;
+      }
+      A.acc_w[w] += wt;
+    }
+    A.acc_el[w] += wt * el;
+    A.acc_el2[w] += wt * el * el;
     A.acc_n[w] += accept ? 1.0 : 0.0;
     A.acc_prop_el[w] += sing ? 0.0 : EL;
   }
+}
+
+// ---------------------------------------------------------------------------
+// Walker-state copies for branching: one workgroup per (src, dst) pair.
+__global__ void __launch_bounds__(256)
+copy_walkers_kernel(CopyArgs A, const int *pairs, int np) {
+  const int b = blockIdx.x;
+  if (b >= np) return;
+  const long src = pairs[2 * b], dst = pairs[2 * b + 1];
+  for (int k = 0; k < A.narr; k++) {
+    const long nw = A.bytes[k] / 4;
+    const uint32_t *s = (const uint32_t *)(A.base[k] + src * A.bytes[k]);
+    uint32_t *d = (uint32_t *)(A.base[k] + dst * A.bytes[k]);
+    for (long i = threadIdx.x; i < nw; i += blockDim.x) d[i] = s[i];
+  }
+}
+
+hipError_t launch_copy_walkers(const CopyArgs &a, const int *pairs, int np, hipStream_t st) {
+  if (np <= 0) return hipSuccess;
+  hipLaunchKernelGGL(copy_walkers_kernel, dim3(np), dim3(256), 0, st, a, pairs, np);
+  return hipGetLastError();
 }
 
 template <typename TS, typename TO>

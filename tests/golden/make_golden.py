@@ -23,7 +23,7 @@ from qmcbeaver_amd import synth  # noqa: E402
 
 DRIVER = os.path.join(ROOT, "oracle", "_ref", "qmcb_ref_driver")
 OUT = os.path.dirname(os.path.abspath(__file__))
-NFIX = 11
+NFIX = 12
 
 
 def parse(raw: np.ndarray, N: int):
@@ -32,7 +32,7 @@ def parse(raw: np.ndarray, N: int):
         "detA": rec[:, 0], "detB": rec[:, 1], "U": rec[:, 2], "e_local": rec[:, 3],
         "kinetic": rec[:, 4], "potential": rec[:, 5], "v_ne": rec[:, 6],
         "v_ee": rec[:, 7], "singular": rec[:, 8], "mod_ratio": rec[:, 9],
-        "log_psi": rec[:, 10],
+        "log_psi": rec[:, 10], "sign_psi": rec[:, 11],
         "grad_lnpsi": rec[:, NFIX:NFIX + 3 * N].reshape(-1, N, 3),
         "mod_drift": rec[:, NFIX + 3 * N:].reshape(-1, N, 3),
     }
