@@ -234,6 +234,57 @@ qmcb_status qmcb_dmc_population(qmcb_ctx *ctx, int32_t cap, int32_t *slots,
 
 qmcb_status qmcb_sync(qmcb_ctx *ctx);
 
+/* ---- per-sweep observable series ----------------------------------------
+ * The reference collects statistics once per electron sweep, at the move of
+ * electron 0 (QMCWalker.cpp:2302-2320), from the estimator
+ * p E_L(R') + (1 - p) E_L(R) of every walker with its weight
+ * (QMCWalker.cpp:2096-2123, 2340-2346), and feeds the walker average into its
+ * accumulated properties with the total weight (QMCRun.cpp:544-576).  The
+ * device steps record one row per such sweep:
+ *   {sum_w w E, sum_w w, sum_w w p, walkers}
+ * so that sample = row[0] / row[1] and weight = row[1].  Rows of the last
+ * qmcb_vmc_steps / qmcb_dmc_steps call; *count = rows recorded (may exceed
+ * cap, then only cap rows are copied). */
+qmcb_status qmcb_sample_series(qmcb_ctx *ctx, int32_t cap, double *rows, int32_t *count);
+
+/* ---- DDDA statistics ----------------------------------------------------
+ * Weighted binary-blocking estimator of the reference (QMCProperty.cpp,
+ * QMCStatistic.cpp): replaces QMCProperty::newSample (:427-478), operator+
+ * (:511-598), reWeight (:492-503), getAverage / getStandardDeviation /
+ * getDecorrDepth / getCorrelationLength (:127-237) and getBlockVariance
+ * (:830-872).  Host-only; needs no device. */
+#define QMCB_DDDA_LEVELS 30
+typedef struct qmcb_property qmcb_property;
+
+typedef struct qmcb_property_summary {
+  int64_t n_samples;
+  double average;
+  double std_dev;            /* block std at the decorrelation depth; 1e300
+                                below 100 samples, 99 without a plateau      */
+  double std_dev_std;        /* its standard error (0 when depth < 0)        */
+  double correlation_length; /* block variance ratio at the depth (or depth) */
+  double serial_variance;    /* variance of the raw samples                  */
+  double skewness, kurtosis; /* at the depth                                 */
+  int32_t decorr_depth;      /* -2: < 100 samples, -1: no plateau yet        */
+  int32_t n_levels;          /* levels holding samples                       */
+} qmcb_property_summary;
+
+qmcb_property *qmcb_property_create(void);
+void qmcb_property_destroy(qmcb_property *p);
+void qmcb_property_zero(qmcb_property *p);
+void qmcb_property_add(qmcb_property *p, double sample, double weight);
+/* weights may be NULL (unit weights) */
+qmcb_status qmcb_property_add_series(qmcb_property *p, int64_t n, const double *samples,
+                                     const double *weights);
+/* dst = dst + src, the reference's operator+ (merge of two accumulations) */
+qmcb_status qmcb_property_merge(qmcb_property *dst, const qmcb_property *src);
+void qmcb_property_reweight(qmcb_property *p, double w);
+qmcb_status qmcb_property_summary_get(const qmcb_property *p, qmcb_property_summary *out);
+/* Blocking level i (block size 2^i): samples, summed weights, average and
+ * block variance (variance / (n - 1)). */
+qmcb_status qmcb_property_level(const qmcb_property *p, int32_t level, int64_t *n_samples,
+                                double *sum_weights, double *average, double *block_variance);
+
 /* Per-kernel timing of the last qmcb_vmc_steps call, measured with HIP
  * events on the context's stream.  names/ms arrays of length cap; returns
  * the number of entries. */

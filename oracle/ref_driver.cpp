@@ -12,6 +12,7 @@
 //   qmcb_ref_driver eval <input.ckmf> <configs.bin> <out.bin>
 //   qmcb_ref_driver seq  <input.ckmf> <moves.bin>   <out.bin>
 //   qmcb_ref_driver vmc  <input.ckmf> <configs.bin> <n_moves> <tau> <seed>
+//   qmcb_ref_driver ddda <series.bin> <split>
 //
 // configs.bin: int32 nconf, int32 N, double R[nconf][N][3]
 // moves.bin:   int32 nwalk, int32 N, int32 nmoves, then per walker
@@ -30,6 +31,7 @@
 #include <sys/time.h>
 
 #include "QMCInput.h"
+#include "QMCProperty.h"
 #include "QMCSCFJastrow.h"
 #include "QMCWalkerData.h"
 
@@ -101,12 +103,66 @@ static double gauss() {
   return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
 }
 
+// The reference's DDDA estimator (QMCProperty.cpp:187-237, 427-478, 511-598,
+// 830-872; QMCStatistic.cpp) over a weighted series.
+//   qmcb_ref_driver ddda <series.bin> <split>
+// series.bin: int32 n, then n x (double sample, double weight).  The samples
+// [0, split) and [split, n) go into two properties that are merged with the
+// reference's operator+; split = n means one property.  Prints one JSON line
+// with the public getters, including getBlockVariance for every level
+// (non-finite values as null).
+static void jnum(double v) {
+  if (std::isfinite(v)) std::printf("%.17g", v);
+  else std::printf("null");
+}
+
+static void print_prop(QMCProperty &p) {
+  std::printf("{\"n\": %lu, \"average\": ", p.getNumberSamples());
+  jnum(p.getAverage());
+  std::printf(", \"std\": ");
+  jnum(p.getStandardDeviation());
+  std::printf(", \"corr_len\": ");
+  jnum(p.getCorrelationLength());
+  std::printf(", \"serial_var\": ");
+  jnum(p.getSeriallyCorrelatedVariance());
+  std::printf(", \"std_std\": ");
+  jnum(p.getStandardDeviationStandardDeviation());
+  std::printf(", \"block_var\": [");
+  for (int i = 0; i < DCL; i++) {
+    if (i) std::printf(", ");
+    jnum(p.getBlockVariance(i));
+  }
+  std::printf("]}\n");
+}
+
+static int ddda_main(int argc, char **argv) {
+  if (argc < 4) return 2;
+  std::FILE *f = std::fopen(argv[2], "rb");
+  if (!f) return 1;
+  int n = 0;
+  if (std::fread(&n, sizeof(int), 1, f) != 1) return 3;
+  std::vector<double> sw(2 * (size_t)n);
+  if (n > 0 && std::fread(&sw[0], sizeof(double), sw.size(), f) != sw.size()) return 4;
+  std::fclose(f);
+  int split = std::atoi(argv[3]);
+  if (split < 0 || split > n) return 5;
+  // static: QMCStatistic leaves its sums uninitialised until the first
+  // sample, and operator+ adds them for empty levels
+  static QMCProperty a, b;
+  for (int i = 0; i < split; i++) a.newSample(sw[2 * i], sw[2 * i + 1]);
+  for (int i = split; i < n; i++) b.newSample(sw[2 * i], sw[2 * i + 1]);
+  QMCProperty m = a + b;
+  print_prop(m);
+  return 0;
+}
+
 int main(int argc, char **argv) {
-  if (argc < 5) {
+  if (argc < 4) {
     std::fprintf(stderr, "usage: %s eval|seq|vmc input.ckmf in out|n_moves [tau seed]\n", argv[0]);
     return 2;
   }
   std::string mode = argv[1];
+  if (mode == "ddda") return ddda_main(argc, argv);
   globalInput.read(std::string(argv[2]));
   QMCSCFJastrow qmf;
   qmf.initialize(&globalInput, 0);
@@ -196,7 +252,8 @@ int main(int argc, char **argv) {
       el[w] = wd[w]->localEnergy;
       F[w] = wd[w]->modifiedGradPsiRatio;
     }
-    double t0 = now(), esum = 0.0, acc = 0.0;
+    double t0 = now(), esum = 0.0, acc = 0.0, sweep_e = 0.0;
+    static QMCProperty energy;  // static: zero-filled sums (see ddda_main)
     double sig = std::sqrt(tau);
     for (int m = 0; m < n_moves; m++) {
       int e = m % N;
@@ -223,6 +280,13 @@ int main(int argc, char **argv) {
             rev += t * t;
           }
         double p = wd[w]->singular ? 0.0 : std::exp(2.0 * (lpn - lp[w]) - (rev - fwd) / (2.0 * tau));
+        if (!(p == p)) p = 0.0;
+        if (e == 0) {
+          // statistics once per sweep, at the electron-0 move, from
+          // p E_L(R') + (1 - p) E_L(R) (QMCWalker.cpp:2096-2104, 2302-2346)
+          const double pc = p > 1.0 ? 1.0 : p;
+          sweep_e += pc > 0.0 ? pc * wd[w]->localEnergy + (1.0 - pc) * el[w] : el[w];
+        }
         if (p > unif()) {
           lp[w] = lpn; el[w] = wd[w]->localEnergy; F[w] = wd[w]->modifiedGradPsiRatio;
           acc += 1.0;
@@ -233,10 +297,18 @@ int main(int argc, char **argv) {
         }
         esum += el[w];
       }
+      if (e == 0) {
+        // walker average with the total weight (QMCRun.cpp:544-576)
+        energy.newSample(sweep_e / nw, (double)nw);
+        sweep_e = 0.0;
+      }
     }
     double dt = now() - t0;
-    std::printf("{\"seconds\": %.6f, \"walker_moves\": %lld, \"mean_e_local\": %.10f, \"acceptance\": %.6f}\n",
-                dt, (long long)nw * n_moves, esum / ((double)nw * n_moves), acc / ((double)nw * n_moves));
+    std::printf("{\"seconds\": %.6f, \"walker_moves\": %lld, \"mean_e_local\": %.10f, "
+                "\"acceptance\": %.6f, \"ddda_samples\": %lu, \"ddda_energy\": %.12f, "
+                "\"ddda_std\": %.12g}\n",
+                dt, (long long)nw * n_moves, esum / ((double)nw * n_moves), acc / ((double)nw * n_moves),
+                energy.getNumberSamples(), energy.getAverage(), energy.getStandardDeviation());
   } else {
     return 2;
   }

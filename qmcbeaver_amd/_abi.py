@@ -71,6 +71,13 @@ class DmcStats(C.Structure):
                 ("e_trial", C.c_double)]
 
 
+class PropertySummary(C.Structure):
+    _fields_ = [("n_samples", C.c_int64), ("average", C.c_double), ("std_dev", C.c_double),
+                ("std_dev_std", C.c_double), ("correlation_length", C.c_double),
+                ("serial_variance", C.c_double), ("skewness", C.c_double),
+                ("kurtosis", C.c_double), ("decorr_depth", C.c_int32), ("n_levels", C.c_int32)]
+
+
 def _pade_struct(row) -> Pade:
     p = Pade()
     p.n_num, p.n_den = int(row[0]), int(row[1])

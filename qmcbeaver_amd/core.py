@@ -13,7 +13,8 @@ from typing import Dict, Optional, Sequence
 import numpy as np
 
 from ._abi import (Options, ResultArrays, Results, SystemBuffers, SystemStruct,
-                   VmcStats, DmcParams, DmcStats, PRECISIONS, STATUS, QMCB_DRIFT_NONE)
+                   VmcStats, DmcParams, DmcStats, PropertySummary, PRECISIONS, STATUS,
+                   QMCB_DRIFT_NONE)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("QMCB_LIB_PATH") or os.path.join(HERE, "libqmcb_amd.so")
@@ -22,7 +23,10 @@ EXPORTS = ["qmcb_abi_version", "qmcb_create", "qmcb_destroy", "qmcb_last_error",
            "qmcb_evaluate", "qmcb_evaluate_device", "qmcb_accept", "qmcb_refresh",
            "qmcb_get_positions", "qmcb_vmc_steps", "qmcb_sync", "qmcb_kernel_times",
            "qmcb_enable_kernel_timing", "qmcb_dmc_init", "qmcb_dmc_steps",
-           "qmcb_dmc_population"]
+           "qmcb_dmc_population", "qmcb_sample_series", "qmcb_property_create",
+           "qmcb_property_destroy", "qmcb_property_zero", "qmcb_property_add",
+           "qmcb_property_add_series", "qmcb_property_merge", "qmcb_property_reweight",
+           "qmcb_property_summary_get", "qmcb_property_level"]
 
 _lib = None
 
@@ -75,6 +79,22 @@ def lib():
                                  C.POINTER(DmcStats)]
     L.qmcb_dmc_population.restype = C.c_int
     L.qmcb_dmc_population.argtypes = [vp, C.c_int32, ip, dp, C.POINTER(C.c_int32)]
+    L.qmcb_sample_series.restype = C.c_int
+    L.qmcb_sample_series.argtypes = [vp, C.c_int32, dp, C.POINTER(C.c_int32)]
+    L.qmcb_property_create.restype = vp
+    L.qmcb_property_create.argtypes = []
+    L.qmcb_property_destroy.argtypes = [vp]
+    L.qmcb_property_zero.argtypes = [vp]
+    L.qmcb_property_add.argtypes = [vp, C.c_double, C.c_double]
+    L.qmcb_property_add_series.restype = C.c_int
+    L.qmcb_property_add_series.argtypes = [vp, C.c_int64, dp, dp]
+    L.qmcb_property_merge.restype = C.c_int
+    L.qmcb_property_merge.argtypes = [vp, vp]
+    L.qmcb_property_reweight.argtypes = [vp, C.c_double]
+    L.qmcb_property_summary_get.restype = C.c_int
+    L.qmcb_property_summary_get.argtypes = [vp, C.POINTER(PropertySummary)]
+    L.qmcb_property_level.restype = C.c_int
+    L.qmcb_property_level.argtypes = [vp, C.c_int32, C.POINTER(C.c_int64), dp, dp, dp]
     _lib = L
     return L
 
@@ -182,6 +202,18 @@ class Context:
         self._check(lib().qmcb_dmc_steps(self._h, first_step, n_steps, C.byref(p),
                                          C.byref(st)), "qmcb_dmc_steps")
         return st
+
+    def sample_series(self) -> np.ndarray:
+        """Per-sweep observable rows of the last vmc_steps/dmc_steps call:
+        [rows, 4] = (sum w E, sum w, sum w p, walkers); the sweep's sample is
+        row[0] / row[1] with weight row[1]."""
+        n = C.c_int32()
+        self._check(lib().qmcb_sample_series(self._h, 0, None, C.byref(n)),
+                    "qmcb_sample_series")
+        rows = np.zeros((max(n.value, 1), 4))
+        self._check(lib().qmcb_sample_series(self._h, n.value, _dp(rows), C.byref(n)),
+                    "qmcb_sample_series")
+        return rows[:n.value].copy()
 
     def dmc_population(self):
         cap = self.max_walkers

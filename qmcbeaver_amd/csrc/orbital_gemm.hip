@@ -12,7 +12,9 @@
 // 16-point tile and MT 16-orbital tiles; each k-step issues MT MFMA
 // 16x16x4 instructions (FP64: v_mfma_f64_16x16x4f64, FP32:
 // v_mfma_f32_16x16x4f32) with lane l supplying A[l%16][l/16] and
-// B[l/16][l%16]; lane l receives D[4(l/16)+i][l%16], i = 0..3.
+// B[l/16][l%16].  Accumulator element i (i = 0..3) of lane l is
+// D[row][l%16] with row = 4(l/16) + i for the FP32 instruction and
+// row = 4i + l/16 for the FP64 one (Mfma<T>::row).
 #include "qmcb_kernels.hpp"
 
 namespace qmcb {
@@ -32,6 +34,8 @@ struct Mfma<double> {
   __device__ static acc_t run(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  // output row of accumulator element i in lane group lk = lane / 16
+  __device__ static constexpr int row(int lk, int i) { return 4 * i + lk; }
 };
 
 template <>
@@ -42,6 +46,7 @@ struct Mfma<float> {
   __device__ static acc_t run(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  __device__ static constexpr int row(int lk, int i) { return 4 * lk + i; }
 };
 
 template <typename TI, typename TO, int MT>
@@ -79,7 +84,7 @@ orbital_gemm_kernel(const TI *__restrict__ Ct, int ldc, int n_orb, const TI *__r
 #pragma unroll
     for (int t = 0; t < MT; t++) {
 #pragma unroll
-      for (int i = 0; i < 4; i++) tile[wave][li][4 * lk + i] = (TO)acc[t][i];
+      for (int i = 0; i < 4; i++) tile[wave][li][M::row(lk, i)] = (TO)acc[t][i];
       __syncthreads();
       const int pp = lane >> 2, oo = (lane & 3) * 4;
       const long p = p0 + pp;
@@ -104,7 +109,7 @@ orbital_gemm_kernel(const TI *__restrict__ Ct, int ldc, int n_orb, const TI *__r
     for (int t = 0; t < MT; t++)
 #pragma unroll
       for (int i = 0; i < 4; i++) {
-        const int orb = o0 + 16 * t + 4 * lk + i;
+        const int orb = o0 + 16 * t + M::row(lk, i);
         if (orb < n_orb) ov[(long)orb * o.s_orb] = (double)acc[t][i];
       }
     return;
@@ -115,7 +120,7 @@ orbital_gemm_kernel(const TI *__restrict__ Ct, int ldc, int n_orb, const TI *__r
   for (int t = 0; t < MT; t++) {
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-      const int orb = o0 + 16 * t + 4 * lk + i;
+      const int orb = o0 + 16 * t + M::row(lk, i);
       if (orb < n_orb) out[base + (long)orb * o.s_orb] = (TO)acc[t][i];
     }
   }

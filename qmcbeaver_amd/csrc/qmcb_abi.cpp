@@ -79,6 +79,10 @@ struct qmcb_ctx {
   uint8_t *d_mask = nullptr;
   // vmc accumulators
   double *acc_el = nullptr, *acc_el2 = nullptr, *acc_n = nullptr, *acc_prop_el = nullptr;
+  // per-sweep observable series (qmcb_sample_series)
+  double *d_obs = nullptr;     // [Wc][3]
+  double *d_series = nullptr;  // [series_cap][4]
+  int series_cap = 0, series_len = 0;
   // dmc population
   double *weight = nullptr, *acc_w = nullptr;
   int *d_active = nullptr, *d_pairs = nullptr;
@@ -558,6 +562,7 @@ qmcb_status create_impl(const qmcb_system *s, const qmcb_options *o, qmcb_ctx *c
   QCHK(dalloc(ctx, &ctx->acc_el2, W));
   QCHK(dalloc(ctx, &ctx->acc_n, W));
   QCHK(dalloc(ctx, &ctx->acc_prop_el, W));
+  QCHK(dalloc(ctx, &ctx->d_obs, 3 * W));
   QCHK(dalloc(ctx, &ctx->weight, W));
   QCHK(dalloc(ctx, &ctx->acc_w, W));
   QCHK(dalloc(ctx, &ctx->d_active, W));
@@ -819,9 +824,30 @@ qmcb_status qmcb_get_positions(qmcb_ctx *ctx, int32_t num, double *R) {
   return QMCB_OK;
 }
 
+// Rows of the observable series a step range records: one per step that
+// moves electron 0.  Grows the device buffer before the step loop.
+static qmcb_status series_reserve(qmcb_ctx *ctx, int64_t first_step, int32_t n_steps) {
+  const int64_t N = ctx->N;
+  const int64_t first = ((first_step + N - 1) / N) * N;  // first multiple of N >= first_step
+  const int64_t last = first_step + n_steps;             // exclusive
+  const int64_t rows = last > first ? (last - first + N - 1) / N : 0;
+  ctx->series_len = 0;
+  if (rows > INT32_MAX / 8) {
+    ctx->err = "too many sweeps in one call for the observable series";
+    return QMCB_INVALID_ARG;
+  }
+  if (rows > ctx->series_cap) {
+    const int cap = (int)std::max<int64_t>(rows, 2 * (int64_t)ctx->series_cap);
+    QCHK(dalloc(ctx, &ctx->d_series, 4 * (size_t)cap));
+    ctx->series_cap = cap;
+  }
+  return QMCB_OK;
+}
+
 qmcb_status qmcb_vmc_steps(qmcb_ctx *ctx, int32_t num, int64_t first_step, int32_t n_steps,
                            double tau, qmcb_vmc_stats *stats) {
-  if (!ctx || num < 1 || num > ctx->Wc || n_steps < 0 || !(tau > 0)) return QMCB_INVALID_ARG;
+  if (!ctx || num < 1 || num > ctx->Wc || n_steps < 0 || first_step < 0 || !(tau > 0))
+    return QMCB_INVALID_ARG;
   if (ctx->opt.drift_mod != QMCB_DRIFT_NONE) {
     ctx->err = "device VMC supports QF_modification_type none";
     return QMCB_INVALID_ARG;
@@ -836,6 +862,10 @@ qmcb_status qmcb_vmc_steps(qmcb_ctx *ctx, int32_t num, int64_t first_step, int32
   QCHK(hipMemsetAsync(ctx->acc_el2, 0, sizeof(double) * num, ctx->stream));
   QCHK(hipMemsetAsync(ctx->acc_n, 0, sizeof(double) * num, ctx->stream));
   QCHK(hipMemsetAsync(ctx->acc_prop_el, 0, sizeof(double) * num, ctx->stream));
+  {
+    qmcb_status rs = series_reserve(ctx, first_step, n_steps);
+    if (rs != QMCB_OK) return rs;
+  }
   qmcb_results none{};
   for (int s = 0; s < n_steps; s++) {
     const long step = first_step + s;
@@ -859,8 +889,12 @@ qmcb_status qmcb_vmc_steps(qmcb_ctx *ctx, int32_t num, int64_t first_step, int32
     if (st != QMCB_OK) return st;
     MoveArgs a = DISPATCH(ctx, move_args(ctx, m, 1, none));
     a.tau = tau;
+    if (m == 0) a.obs = ctx->d_obs;
     st = DISPATCH(ctx, sm(ctx, a, nullptr, num));
     if (st != QMCB_OK) return st;
+    if (m == 0)
+      TIMED("series", launch_series_reduce(ctx->d_obs, num, ctx->d_series + 4L * ctx->series_len++,
+                                           ctx->stream));
     // FP64 re-inversion of a spin's Slater matrices after refresh_interval
     // Sherman-Morrison updates of that spin (all walkers move in lockstep)
     const int sp = m < ctx->n[0] ? 0 : 1;
@@ -950,7 +984,7 @@ qmcb_status qmcb_dmc_init(qmcb_ctx *ctx, int32_t num) {
 
 qmcb_status qmcb_dmc_steps(qmcb_ctx *ctx, int64_t first_step, int32_t n_steps,
                            const qmcb_dmc_params *prm, qmcb_dmc_stats *stats) {
-  if (!ctx || !prm || n_steps < 0 || !(prm->tau > 0)) return QMCB_INVALID_ARG;
+  if (!ctx || !prm || n_steps < 0 || first_step < 0 || !(prm->tau > 0)) return QMCB_INVALID_ARG;
   if (!ctx->dmc_ready) {
     ctx->err = "qmcb_dmc_init first";
     return QMCB_BAD_STATE;
@@ -969,6 +1003,10 @@ qmcb_status qmcb_dmc_steps(qmcb_ctx *ctx, int64_t first_step, int32_t n_steps,
   if (!ctx->have_e_trial) {
     ctx->e_trial = prm->e_trial;
     ctx->have_e_trial = true;
+  }
+  {
+    qmcb_status rs = series_reserve(ctx, first_step, n_steps);
+    if (rs != QMCB_OK) return rs;
   }
   const int target = prm->target_walkers > 0 ? prm->target_walkers : (int)ctx->active.size();
   std::mt19937_64 rng(ctx->opt.seed ^ (0x9E3779B97F4A7C15ULL * (uint64_t)(first_step + 1)));
@@ -1008,8 +1046,12 @@ qmcb_status qmcb_dmc_steps(qmcb_ctx *ctx, int64_t first_step, int32_t n_steps,
     a.e_trial = ctx->e_trial;
     a.tau_eff = prm->tau;
     a.reweight = (m == 0 && step >= prm->reweight_after) ? 1 : 0;
+    if (m == 0) a.obs = ctx->d_obs;
     sst = DISPATCH(ctx, sm(ctx, a, ctx->d_active, pop));
     if (sst != QMCB_OK) return sst;
+    if (m == 0)
+      TIMED("series", launch_series_reduce(ctx->d_obs, pop, ctx->d_series + 4L * ctx->series_len++,
+                                           ctx->stream));
     moves += pop;
     const int sp = m < ctx->n[0] ? 0 : 1;
     if (ctx->opt.refresh_interval > 0 && ++ctx->sm_updates[sp] >= ctx->opt.refresh_interval) {
@@ -1129,6 +1171,19 @@ qmcb_status qmcb_dmc_population(qmcb_ctx *ctx, int32_t cap, int32_t *slots, doub
     if (slots) slots[i] = ctx->active[i];
     if (weights) weights[i] = wv[ctx->active[i]];
   }
+  return QMCB_OK;
+}
+
+qmcb_status qmcb_sample_series(qmcb_ctx *ctx, int32_t cap, double *rows, int32_t *count) {
+  if (!ctx || cap < 0 || (cap > 0 && !rows)) return QMCB_INVALID_ARG;
+  QCHK(hipSetDevice(ctx->device));
+  if (count) *count = ctx->series_len;
+  const int n = std::min(cap, ctx->series_len);
+  if (n > 0) {
+    QCHK(hipMemcpyAsync(rows, ctx->d_series, sizeof(double) * 4 * n, hipMemcpyDeviceToHost,
+                        ctx->stream));
+  }
+  QCHK(hipStreamSynchronize(ctx->stream));
   return QMCB_OK;
 }
 

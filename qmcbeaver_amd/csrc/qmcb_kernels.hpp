@@ -110,7 +110,14 @@ struct MoveArgs {
   double *acc_w;         // [Wc] sum of weights over the steps
   double e_trial, tau_eff;
   int reweight;          // reweight this step (electron 0 moved)
+  // observable sample of this step (null except once per sweep):
+  // obs[3 b + {0,1,2}] = {p E_L(R') + (1 - p) E_L(R), weight, p}
+  double *obs;
 };
+
+// One row {sum w E, sum w, sum w p, walkers} of the per-sweep observable
+// series from obs[3 * num] (single workgroup, fixed summation order).
+hipError_t launch_series_reduce(const double *obs, int num, double *row, hipStream_t st);
 
 // copy whole walker states slot -> slot (DMC branching)
 constexpr int kMaxCopyArrays = 40;

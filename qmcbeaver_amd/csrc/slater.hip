@@ -776,7 +776,49 @@ sm_move_kernel(MoveArgs A, const int *walkers, int num) {
     A.acc_el2[w] += wt * el * el;
     A.acc_n[w] += accept ? 1.0 : 0.0;
     A.acc_prop_el[w] += sing ? 0.0 : EL;
+    if (A.obs) {
+      // estimator p E_L(R') + (1 - p) E_L(R) (QMCWalker.cpp:2096-2104)
+      A.obs[3L * b] = p > 0.0 ? p * EL + (1.0 - p) * el_old : el_old;
+      A.obs[3L * b + 1] = wt;
+      A.obs[3L * b + 2] = p;
+    }
   }
+}
+
+// ---------------------------------------------------------------------------
+// Per-sweep observable row: one workgroup sums the walkers in a fixed order
+// (strided per thread, then a tree over the waves), so rows are reproducible.
+__global__ void __launch_bounds__(1024)
+series_reduce_kernel(const double *obs, int num, double *row) {
+  __shared__ double red[3][16];
+  const int t = threadIdx.x;
+  double se = 0.0, sw = 0.0, sp = 0.0;
+  for (int b = t; b < num; b += 1024) {
+    const double e = obs[3L * b], w = obs[3L * b + 1], p = obs[3L * b + 2];
+    se += w * e;
+    sw += w;
+    sp += w * p;
+  }
+  se = wave_sum(se);
+  sw = wave_sum(sw);
+  sp = wave_sum(sp);
+  if ((t & 63) == 0) {
+    red[0][t >> 6] = se;
+    red[1][t >> 6] = sw;
+    red[2][t >> 6] = sp;
+  }
+  __syncthreads();
+  if (t < 3) {
+    double s = 0.0;
+    for (int q = 0; q < 16; q++) s += red[t][q];
+    row[t] = s;
+  }
+  if (t == 3) row[3] = (double)num;
+}
+
+hipError_t launch_series_reduce(const double *obs, int num, double *row, hipStream_t st) {
+  hipLaunchKernelGGL(series_reduce_kernel, dim3(1), dim3(1024), 0, st, obs, num, row);
+  return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------

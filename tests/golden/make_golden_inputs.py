@@ -21,6 +21,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__)
 sys.path.insert(0, ROOT)
 from qmcbeaver_amd import ckmf  # noqa: E402
 from tests.golden.make_golden import DRIVER, parse  # noqa: E402
+from tests.inputs_util import configs  # noqa: E402
 
 OUT = os.path.dirname(os.path.abspath(__file__))
 REF = "/root/reference/QMcBeaver"
@@ -32,16 +33,6 @@ INPUTS = [
 OVERRIDES = {"replace_electron_nucleus_cusps": "0", "one_e_per_iter": "0",
              "walkers_per_pass": "1", "QF_modification_type": "none",
              "use_available_checkpoints": "0"}
-
-
-def configs(packed, n, seed):
-    rng = np.random.default_rng(seed)
-    Z = np.maximum(packed["nuc_z"].astype(int), 1)
-    order = np.repeat(np.arange(packed["n_nuc"]), Z)
-    N = packed["n_up"] + packed["n_dn"]
-    nuc = np.array([order[(2 * k) % len(order)] for k in range(packed["n_up"])] +
-                   [order[(2 * k + 1) % len(order)] for k in range(packed["n_dn"])])
-    return packed["nuc_xyz"][nuc][None] + 0.5 * rng.normal(size=(n, N, 3))
 
 
 def main():

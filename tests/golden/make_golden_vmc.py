@@ -1,0 +1,64 @@
+"""VMC energy of the reference's H2 6-31G** example (examples/H2_hf_631Gss-vmc.ckmf,
+kept here as tests/golden/h2_631gss.ckmf.gz) with the reference evaluate and
+the reference DDDA estimator, for the statistical parity test of the device
+VMC.  Runs oracle/_ref/qmcb_ref_driver in its vmc mode: importance-sampled
+single-electron moves with the Green's function ratio, rejection by
+re-evaluation, one statistics sample per electron sweep from
+p E_L(R') + (1 - p) E_L(R) averaged over walkers, fed to QMCProperty.
+Authoring container only (needs /root/reference):
+
+    make -f oracle/ref.mk && python tests/golden/make_golden_vmc.py
+"""
+from __future__ import annotations
+
+import gzip
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+from qmcbeaver_amd import ckmf  # noqa: E402
+from tests.inputs_util import configs  # noqa: E402
+
+DRIVER = os.path.join(ROOT, "oracle", "_ref", "qmcb_ref_driver")
+OUT = os.path.dirname(os.path.abspath(__file__))
+OVERRIDES = {"replace_electron_nucleus_cusps": "0", "one_e_per_iter": "1",
+             "walkers_per_pass": "1", "QF_modification_type": "none",
+             "use_available_checkpoints": "0"}
+RUN = dict(walkers=2048, sweeps=6000, tau=0.05, seed=2024, config_seed=778)
+
+
+def main():
+    if not os.path.exists(DRIVER):
+        sys.exit("build the reference driver first: make -f oracle/ref.mk")
+    with tempfile.TemporaryDirectory() as tmp:
+        raw = os.path.join(tmp, "h2.ckmf")
+        with gzip.open(os.path.join(OUT, "h2_631gss.ckmf.gz"), "rb") as g, open(raw, "wb") as f:
+            f.write(g.read())
+        ck = os.path.join(tmp, "in.ckmf")
+        ckmf.write_with_flags(raw, ck, OVERRIDES)
+        packed, flags, _ = ckmf.read_ckmf(ck)
+        R = configs(packed, RUN["walkers"], RUN["config_seed"])
+        cf = os.path.join(tmp, "cfg.bin")
+        with open(cf, "wb") as f:
+            np.array(R.shape[:2], dtype=np.int32).tofile(f)
+            R.tofile(f)
+        N = R.shape[1]
+        out = subprocess.run([DRIVER, "vmc", ck, cf, str(RUN["sweeps"] * N), str(RUN["tau"]),
+                              str(RUN["seed"])], check=True, capture_output=True, text=True,
+                             cwd=tmp).stdout
+        res = json.loads(out.strip().splitlines()[-1])
+    res.pop("seconds", None)
+    rec = dict(RUN, input="h2_631gss", overrides=OVERRIDES, reference=res)
+    with open(os.path.join(OUT, "vmc_h2_631gss.json"), "w") as f:
+        json.dump(rec, f, indent=1)
+    print(rec)
+
+
+if __name__ == "__main__":
+    main()

@@ -25,3 +25,15 @@ def load(name):
         ckmf.write_with_flags(raw, ck, OVERRIDES)
         packed, flags, info = ckmf.read_ckmf(ck)
     return packed, np.load(os.path.join(GOLDEN, name + ".npz")), flags
+
+
+def configs(packed, n, seed):
+    """n configurations: electron k of each spin near nucleus order[2k (+1)]
+    (nuclei repeated Z times), displaced by N(0, 0.5^2)."""
+    rng = np.random.default_rng(seed)
+    Z = np.maximum(packed["nuc_z"].astype(int), 1)
+    order = np.repeat(np.arange(packed["n_nuc"]), Z)
+    nuc = np.array([order[(2 * k) % len(order)] for k in range(packed["n_up"])] +
+                   [order[(2 * k + 1) % len(order)] for k in range(packed["n_dn"])])
+    N = packed["n_up"] + packed["n_dn"]
+    return packed["nuc_xyz"][nuc][None] + 0.5 * rng.normal(size=(n, N, 3))
