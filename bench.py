@@ -186,7 +186,20 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--accuracy", type=int, default=10000,
                     help="configurations for the mixed-vs-FP64 error report (0 = skip)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy three times (the "
+                         "reference kernel for scaling the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": x.numel() * 8, "copies": 3}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: the walker-evaluation path "
                                    "is measured at N=1 only (see DESIGN.md for how it shards)"}))

@@ -100,7 +100,8 @@ typedef struct qmcb_options {
   double umrigar_a;         /* umrigar93_equalelectrons_parameter              */
   uint64_t seed;            /* device RNG seed (vmc entry points)              */
   int32_t device;           /* HIP device ordinal                              */
-  int32_t reserved;
+  int32_t replace_cusps;    /* 1: electron-nucleus cusp replacement of the
+                               orbitals (replace_electron_nucleus_cusps 1)     */
 } qmcb_options;
 
 /* Caller-owned outputs, one row per evaluated walker.  Any pointer may be
@@ -233,6 +234,28 @@ qmcb_status qmcb_dmc_population(qmcb_ctx *ctx, int32_t cap, int32_t *slots,
                                 double *weights, int32_t *count);
 
 qmcb_status qmcb_sync(qmcb_ctx *ctx);
+
+/* ---- electron-nucleus cusp replacement ---------------------------------
+ * With qmcb_options.replace_cusps = 1 the s-type part of every orbital near
+ * every nucleus is replaced inside a radius rc by C + sgn0 exp(alpha(r)), the
+ * reference's replace_electron_nucleus_cusps 1 (QMCElectronNucleusCusp.cpp:
+ * 46-177 fit, :587-640 replaceCusps; QMCSlater.cpp:88-93, 864-871).
+ * qmcb_cusp_fit returns the fitted parameters of one spin's orbitals (host
+ * only, no device needed): out[n_nuc][n_orbitals of the spin]; rc = 0 where
+ * an orbital has no s-type part on a nucleus. */
+typedef struct qmcb_cusp_orbital {
+  double rc;        /* radius of correction (0: no replacement)               */
+  double C;         /* constant shift                                          */
+  double sgn0;      /* sign of the exponential                                 */
+  double alpha[5];  /* alpha(r) = sum alpha[i] r^i                             */
+  double phi0;      /* replacement value at the nucleus                        */
+  double n0;        /* the orbital at the nucleus from the other atoms         */
+  double zeff;      /* effective charge of the replacement's local energy      */
+  double sigma_sq;  /* largest squared deviation from the ideal curve          */
+  double ideal0;    /* constant term of the ideal curve                        */
+} qmcb_cusp_orbital;
+
+qmcb_status qmcb_cusp_fit(const qmcb_system *sys, int32_t spin, qmcb_cusp_orbital *out);
 
 /* ---- per-sweep observable series ----------------------------------------
  * The reference collects statistics once per electron sweep, at the move of

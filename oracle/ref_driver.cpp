@@ -163,6 +163,9 @@ int main(int argc, char **argv) {
   }
   std::string mode = argv[1];
   if (mode == "ddda") return ddda_main(argc, argv);
+  // full precision for the cusp-replacement parameters the reference prints
+  // with print_replacement_orbitals 1 (QMCElectronNucleusCusp.cpp:163-168)
+  std::clog.precision(17);
   globalInput.read(std::string(argv[2]));
   QMCSCFJastrow qmf;
   qmf.initialize(&globalInput, 0);

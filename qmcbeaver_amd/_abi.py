@@ -41,7 +41,7 @@ class Options(C.Structure):
     _fields_ = [("precision", C.c_int32), ("max_walkers", C.c_int32),
                 ("drift_mod", C.c_int32), ("refresh_interval", C.c_int32),
                 ("dt", C.c_double), ("umrigar_a", C.c_double),
-                ("seed", C.c_uint64), ("device", C.c_int32), ("reserved", C.c_int32)]
+                ("seed", C.c_uint64), ("device", C.c_int32), ("replace_cusps", C.c_int32)]
 
 
 class Results(C.Structure):
@@ -49,6 +49,12 @@ class Results(C.Structure):
                 ("mod_drift", _dp), ("lap_ratio", _dp), ("e_local", _dp),
                 ("kinetic", _dp), ("potential", _dp), ("v_ne", _dp), ("v_ee", _dp),
                 ("det_ratio", _dp), ("jastrow_u", _dp), ("singular", _i8p)]
+
+
+class CuspOrbital(C.Structure):
+    _fields_ = [("rc", C.c_double), ("C", C.c_double), ("sgn0", C.c_double),
+                ("alpha", C.c_double * 5), ("phi0", C.c_double), ("n0", C.c_double),
+                ("zeff", C.c_double), ("sigma_sq", C.c_double), ("ideal0", C.c_double)]
 
 
 class VmcStats(C.Structure):

@@ -13,8 +13,8 @@ from typing import Dict, Optional, Sequence
 import numpy as np
 
 from ._abi import (Options, ResultArrays, Results, SystemBuffers, SystemStruct,
-                   VmcStats, DmcParams, DmcStats, PropertySummary, PRECISIONS, STATUS,
-                   QMCB_DRIFT_NONE)
+                   VmcStats, DmcParams, DmcStats, PropertySummary, CuspOrbital, PRECISIONS,
+                   STATUS, QMCB_DRIFT_NONE)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("QMCB_LIB_PATH") or os.path.join(HERE, "libqmcb_amd.so")
@@ -26,7 +26,7 @@ EXPORTS = ["qmcb_abi_version", "qmcb_create", "qmcb_destroy", "qmcb_last_error",
            "qmcb_dmc_population", "qmcb_sample_series", "qmcb_property_create",
            "qmcb_property_destroy", "qmcb_property_zero", "qmcb_property_add",
            "qmcb_property_add_series", "qmcb_property_merge", "qmcb_property_reweight",
-           "qmcb_property_summary_get", "qmcb_property_level"]
+           "qmcb_property_summary_get", "qmcb_property_level", "qmcb_cusp_fit"]
 
 _lib = None
 
@@ -79,6 +79,8 @@ def lib():
                                  C.POINTER(DmcStats)]
     L.qmcb_dmc_population.restype = C.c_int
     L.qmcb_dmc_population.argtypes = [vp, C.c_int32, ip, dp, C.POINTER(C.c_int32)]
+    L.qmcb_cusp_fit.restype = C.c_int
+    L.qmcb_cusp_fit.argtypes = [C.POINTER(SystemStruct), C.c_int32, C.POINTER(CuspOrbital)]
     L.qmcb_sample_series.restype = C.c_int
     L.qmcb_sample_series.argtypes = [vp, C.c_int32, dp, C.POINTER(C.c_int32)]
     L.qmcb_property_create.restype = vp
@@ -99,6 +101,23 @@ def lib():
     return L
 
 
+def cusp_fit(packed: Dict, spin: int = 0) -> np.ndarray:
+    """Fitted cusp-replacement parameters of one spin's orbitals (host code,
+    no device): structured array [n_nuc][n_orbitals] with fields rc, C,
+    sgn0, alpha[5], phi0, n0, zeff, sigma_sq, ideal0."""
+    buf = SystemBuffers(packed)
+    nn = int(packed["n_nuc"])
+    norb = int(packed["n_up"] if spin == 0 else packed["n_dn"])
+    arr = (CuspOrbital * max(nn * norb, 1))()
+    st = lib().qmcb_cusp_fit(C.byref(buf.struct), spin, arr)
+    if st != 0:
+        raise QmcbError(f"qmcb_cusp_fit: {STATUS.get(st, st)}")
+    dt = np.dtype([("rc", "f8"), ("C", "f8"), ("sgn0", "f8"), ("alpha", "f8", 5), ("phi0", "f8"),
+                   ("n0", "f8"), ("zeff", "f8"), ("sigma_sq", "f8"), ("ideal0", "f8")])
+    out = np.frombuffer(bytes(arr), dtype=dt)[:nn * norb].copy()
+    return out.reshape(nn, norb)
+
+
 def _ip(a):
     return a.ctypes.data_as(C.POINTER(C.c_int32))
 
@@ -117,7 +136,8 @@ class Context:
 
     def __init__(self, packed: Dict, *, precision: str = "fp64", max_walkers: int = 1,
                  drift_mod: int = QMCB_DRIFT_NONE, dt: float = 0.1, umrigar_a: float = 0.5,
-                 refresh_interval: int = 0, seed: int = 12345, device: int = 0):
+                 refresh_interval: int = 0, seed: int = 12345, device: int = 0,
+                 replace_cusps: bool = False):
         self._buf = SystemBuffers(packed)
         self.N = self._buf.N
         self.n_up = int(packed["n_up"])
@@ -131,6 +151,7 @@ class Context:
         o.umrigar_a = umrigar_a
         o.seed = seed
         o.device = device
+        o.replace_cusps = 1 if replace_cusps else 0
         self._opt = o
         h = C.c_void_p()
         st = lib().qmcb_create(C.byref(self._buf.struct), C.byref(o), C.byref(h))

@@ -16,6 +16,7 @@
 #include <string>
 #include <vector>
 
+#include "qmcb_cusp.hpp"
 #include "qmcb_kernels.hpp"
 
 using namespace qmcb;
@@ -79,6 +80,9 @@ struct qmcb_ctx {
   uint8_t *d_mask = nullptr;
   // vmc accumulators
   double *acc_el = nullptr, *acc_el2 = nullptr, *acc_n = nullptr, *acc_prop_el = nullptr;
+  // electron-nucleus cusp replacement (per spin)
+  bool cusp = false;
+  CuspDev cuspd[2] = {};
   // per-sweep observable series (qmcb_sample_series)
   double *d_obs = nullptr;     // [Wc][3]
   double *d_series = nullptr;  // [series_cap][4]
@@ -233,6 +237,10 @@ struct Engine {
       const int n = ctx->n[sp];
       if (n == 0) continue;
       const int chunk = std::max<long>(1, std::min<long>(num, ctx->ldp / n));
+      if (!d_map && chunk < num) {
+        ctx->err = "all-electron evaluate in chunks needs a walker map";
+        return QMCB_INVALID_ARG;
+      }
       for (int c0 = 0; c0 < num; c0 += chunk) {
         const int cw = std::min(chunk, num - c0);
         PointSet pts{ctx->R + 3L * ctx->start[sp], (long)cw * n, n, 3L * ctx->N, 3,
@@ -245,6 +253,8 @@ struct Engine {
                                                           (const TX *)ctx->X, ctx->kpad, ctx->ldp,
                                                           (long)cw * n, ctx->kpad, go,
                                                           ctx->stream)));
+        if (ctx->cusp)
+          TIMED("cusp", (launch_cusp<TS>(ctx->cuspd[sp], pts, go, ctx->stream)));
       }
       SlaterArgs sa = slater_args(ctx, sp);
       TIMED("lu_inverse", launch_lu_inverse<TS>(sa, d_map, num, ctx->stream));
@@ -294,6 +304,7 @@ struct Engine {
     TIMED("orbital_gemm", (launch_orbital_gemm<TX, TO>((const TX *)ctx->Ct[sp], ctx->npad[sp], n,
                                                       (const TX *)ctx->X, ctx->kpad, ctx->ldp,
                                                       (long)num, ctx->kpad, go, ctx->stream)));
+    if (ctx->cusp) TIMED("cusp", (launch_cusp<TO>(ctx->cuspd[sp], pts, go, ctx->stream)));
     return QMCB_OK;
   }
 
@@ -520,6 +531,38 @@ qmcb_status create_impl(const qmcb_system *s, const qmcb_options *o, qmcb_ctx *c
       for (size_t i = 0; i < cnt; i++) cf[i] = (float)ct[i];
       QCHK(hipMemcpyAsync(ctx->Ct[sp], cf.data(), cnt * 4, hipMemcpyHostToDevice, ctx->stream));
       QCHK(hipStreamSynchronize(ctx->stream));
+    }
+  }
+
+  // ---- cusp replacement tables (host fit, device copy)
+  if (o->replace_cusps) {
+    ctx->cusp = true;
+    for (int sp = 0; sp < 2; sp++) {
+      const int n = ctx->n[sp];
+      if (n == 0) continue;
+      CuspTable t;
+      std::string err;
+      if (!fit_cusps(s, sp == 0 ? s->mo_up : s->mo_dn, n, t, err)) {
+        ctx->err = err;
+        return QMCB_INVALID_ARG;
+      }
+      std::vector<double> orb8((size_t)t.nn * n * 8);
+      for (size_t i = 0; i < t.orb.size(); i++) {
+        const qmcb_cusp_orbital &q = t.orb[i];
+        double *f = &orb8[i * 8];
+        f[0] = q.rc; f[1] = q.C; f[2] = q.sgn0;
+        for (int k = 0; k < 5; k++) f[3 + k] = q.alpha[k];
+      }
+      CuspDev &c = ctx->cuspd[sp];
+      c.nn = t.nn;
+      c.norb = n;
+      c.ns = (int)t.s_alpha.size();
+      c.nuc_xyz = d.nuc_xyz;
+      QCHK(upload(ctx, &c.rc_max, t.rc_max.data(), t.rc_max.size()));
+      QCHK(upload(ctx, &c.s_off, t.s_off.data(), t.s_off.size()));
+      QCHK(upload(ctx, &c.s_alpha, t.s_alpha.data(), t.s_alpha.size()));
+      QCHK(upload(ctx, &c.s_coef, t.s_coef.data(), t.s_coef.size()));
+      QCHK(upload(ctx, &c.orb8, orb8.data(), orb8.size()));
     }
   }
 

@@ -36,6 +36,11 @@ struct GemmOut {
   long s_walker_val;
 };
 
+// cusp replacement added to GEMM output rows (device tables in qmcb_cusp.hpp)
+struct CuspDev;
+template <typename TO>
+hipError_t launch_cusp(const CuspDev &c, const PointSet &pts, const GemmOut &o, hipStream_t st);
+
 // scatter batch rows of positions into resident slots: R[map[b]] = Rin[b]
 hipError_t launch_scatter_rows(double *R, const double *Rin, const int *map, int num, int row_len,
                                hipStream_t st);
