@@ -107,6 +107,8 @@ def run(ck, R, moves, tmp):
     r = subprocess.run([DRIVER, "eval", ck, cf, of], check=True, cwd=tmp, capture_output=True,
                        text=True)
     ev = parse_eval(np.fromfile(of, dtype=np.float64), N)
+    if moves.shape[0] == 0:
+        return r.stderr, ev, {}
     mf = os.path.join(tmp, "mv.bin")
     nw, nm = moves.shape[:2]
     with open(mf, "wb") as f:
@@ -142,7 +144,11 @@ def moves_into_cusps(packed, R0, nm, seed):
 def main():
     if not os.path.exists(DRIVER):
         sys.exit("build the reference driver first: make -f oracle/ref.mk")
-    cases = [("h2_631gss", 32, 2, 24), ("hmx", 4, 1, 16), ("cfg1", 8, 2, 30)]
+    # name, configurations, sequence walkers, sequence moves.  HMX has no
+    # move sequence: its input lists more orbitals than electrons, where the
+    # reference's one_e_per_iter row copy (Array2D::setRows, Array2D.h:964-987,
+    # used at QMCSlater.cpp:486-490) reads with the wrong stride.
+    cases = [("h2_631gss", 32, 2, 24), ("hmx", 4, 0, 0), ("cfg1", 8, 2, 30)]
     with tempfile.TemporaryDirectory() as tmp:
         for name, n, nsw, nmv in cases:
             ck = os.path.join(tmp, "in.ckmf")

