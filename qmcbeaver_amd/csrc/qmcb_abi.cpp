@@ -1123,6 +1123,7 @@ qmcb_status qmcb_dmc_steps(qmcb_ctx *ctx, int64_t first_step, int32_t n_steps,
         ctx->free_slots.pop_back();
         pairs.push_back(w);
         pairs.push_back(d);
+        ctx->initialized[d] = 1;  // the copy carries a complete resident state
         wts[d] = wt / 2.0;
         next.push_back(w);
         next.push_back(d);

@@ -73,6 +73,9 @@ def test_cusp_single_electron_sequence_matches_reference(name):
 
 
 def test_cusp_mixed_precision_envelope():
+    """FP32 derivative storage near nuclei: these configurations put
+    electrons inside the correction radii, where |E_L| reaches 10^3-10^4 Ha,
+    so the bar is relative: |dE_L| <= 1e-6 max(|E_L|, N)."""
     g = np.load(os.path.join(GOLDEN, "cusp_cfg1.npz"))
     pk = packed_of("cfg1")
     R = g["R"]
@@ -80,5 +83,5 @@ def test_cusp_mixed_precision_envelope():
     print("case cusp fp32store cfg1", flush=True)
     with Context(pk, precision="fp32store", max_walkers=R.shape[0], replace_cusps=True) as ctx:
         res = ctx.evaluate(np.arange(R.shape[0]), R)
-    el = np.abs(res["e_local"] - g["ev_e_local"]) / N
+    el = np.abs(res["e_local"] - g["ev_e_local"]) / np.maximum(np.abs(g["ev_e_local"]), N)
     assert el.max() < 1e-6, el.max()
