@@ -64,7 +64,41 @@ orbital_gemm_kernel(const TI *__restrict__ Ct, int ldc, int n_orb, const TI *__r
   acc_t acc[MT];
 #pragma unroll
   for (int t = 0; t < MT; t++) acc[t] = acc_t{0, 0, 0, 0};
-  if (live) {
+  if (live && sizeof(TI) == 8) {
+    // FP64: k advances in super-steps of 8 (two MFMA k-steps); the operands of
+    // the next super-step are loaded before the MFMAs of the current one, so
+    // the L2 latency of the coefficient tile overlaps the matrix-core work
+    // (kpad is a multiple of 8; padded rows of Ct and X are zero).  Measured
+    // 10% faster for FP64 and slower for FP32, whose MFMA steps are half as
+    // long and whose doubled register footprint costs occupancy.
+    const TI *__restrict__ xp = X + (long)comp * ldk * ldp + (long)lk * ldp + p0 + li;
+    const TI *__restrict__ cp = Ct + (long)lk * ldc + o0 + li;
+    TI a0[2][MT], b0[2], a1[2][MT], b1[2];
+    auto load = [&](int k0, TI (&a)[2][MT], TI (&b)[2]) {
+#pragma unroll
+      for (int s = 0; s < 2; s++) {
+        const long k = k0 + 4 * s;
+        b[s] = xp[k * ldp];
+#pragma unroll
+        for (int t = 0; t < MT; t++) a[s][t] = cp[k * ldc + 16 * t];
+      }
+    };
+    auto step = [&](const TI (&a)[2][MT], const TI (&b)[2]) {
+#pragma unroll
+      for (int s = 0; s < 2; s++)
+#pragma unroll
+        for (int t = 0; t < MT; t++) acc[t] = M::run(a[s][t], b[s], acc[t]);
+    };
+    load(0, a0, b0);
+    int k0 = 0;
+    for (; k0 + 16 <= kpad; k0 += 16) {
+      load(k0 + 8, a1, b1);
+      step(a0, b0);
+      if (k0 + 16 < kpad) load(k0 + 16, a0, b0);
+      step(a1, b1);
+    }
+    if (k0 < kpad) step(a0, b0);  // kpad = 8 (mod 16): one super-step left
+  } else if (live) {
     const TI *__restrict__ xp = X + (long)comp * ldk * ldp + (long)lk * ldp + p0 + li;
     const TI *__restrict__ cp = Ct + (long)lk * ldc + o0 + li;
     for (int k0 = 0; k0 < kpad; k0 += 4) {

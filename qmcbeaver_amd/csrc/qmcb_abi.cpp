@@ -21,6 +21,7 @@
 
 using namespace qmcb;
 
+
 namespace {
 
 thread_local std::string g_create_error;
@@ -207,6 +208,7 @@ void tm_collect(qmcb_ctx *ctx) {
       return QMCB_DEVICE_ERROR;         \
     }                                   \
   } while (0)
+
 
 SlaterArgs slater_args(qmcb_ctx *ctx, int sp) {
   SlaterArgs a{};
@@ -448,7 +450,7 @@ qmcb_status create_impl(const qmcb_system *s, const qmcb_options *o, qmcb_ctx *c
   ctx->nn = s->n_nuc;
   ctx->B = s->n_basis;
   ctx->nprim = s->n_prim;
-  ctx->kpad = ((s->n_basis + 3) / 4) * 4;
+  ctx->kpad = ((s->n_basis + 7) / 8) * 8;  // GEMM k super-steps of 8
   ctx->Wc = o->max_walkers;
   const bool f64 = o->precision == QMCB_FP64;
   const bool store32 = o->precision == QMCB_MIXED_F32STORE;
@@ -457,6 +459,7 @@ qmcb_status create_impl(const qmcb_system *s, const qmcb_options *o, qmcb_ctx *c
   QCHK(hipSetDevice(o->device));
   ctx->device = o->device;
   QCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+
 
   // ---- system tables
   DevSystem &d = ctx->dsys;
@@ -751,6 +754,7 @@ void qmcb_destroy(qmcb_ctx *ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (void *p : ctx->allocs) (void)hipFree(p);
   for (hipEvent_t e : ctx->tm.pool) (void)hipEventDestroy(e);
+
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }

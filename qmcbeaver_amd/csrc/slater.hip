@@ -442,7 +442,7 @@ __device__ inline double drift_factor_unequal(const double g[3], const double x[
 // inverse element on the fly and accumulates w, T1, T2 per column (9 registers).
 // Thread mapping: c = t % NC (electron column), sl = t / NC (row slice).
 #ifndef QMCB_SM_UNROLL
-#define QMCB_SM_UNROLL 4
+#define QMCB_SM_UNROLL 2
 #endif
 #ifndef QMCB_SM_MINWAVES
 #define QMCB_SM_MINWAVES 1
@@ -450,6 +450,7 @@ __device__ inline double drift_factor_unequal(const double g[3], const double x[
 #ifndef QMCB_SM_NT
 #define QMCB_SM_NT 1
 #endif
+
 
 // streamed-once operands: optionally non-temporal loads
 template <typename T>
@@ -517,22 +518,62 @@ sm_move_kernel(MoveArgs A, const int *walkers, int num) {
   const TS *dm = (const TS *)A.s.dmat + (long)w * kDer * nn2;
   double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // w, T1[4], T2[4]
   if (c < n) {
-    double *ip = inv + c;
-    const TS *d1 = dm + c, *d2 = dm + 1 * nn2 + c, *d3 = dm + 2 * nn2 + c,
-             *d4 = dm + 3 * nn2 + c;
-#pragma unroll QMCB_SM_UNROLL
-    for (int o = o_beg; o < o_end; o++) {
-      const long off = (long)o * n;
-      double a = ld_stream(ip + off);
-      const double x = (double)ld_stream(d1 + off), y = (double)ld_stream(d2 + off),
-                   z = (double)ld_stream(d3 + off), l = (double)ld_stream(d4 + off);
+    const TS *__restrict__ d1 = dm + c;
+    const TS *__restrict__ d2 = dm + 1 * nn2 + c;
+    const TS *__restrict__ d3 = dm + 2 * nn2 + c;
+    const TS *__restrict__ d4 = dm + 3 * nn2 + c;
+    // one row of the pass: the current inverse element from A_0 and the
+    // pending terms, then the nine accumulators
+    auto row = [&](int o, double a, double x, double y, double z, double l) {
 #pragma unroll
       for (int q = 0; q < KD; q++) a -= ub[q][o] * zc[q];
-      if (flush) ip[off] = a;
       const double cr = colr[o];
       acc[0] += v[0][o] * a;
       acc[1] += x * a; acc[2] += y * a; acc[3] += z * a; acc[4] += l * a;
       acc[5] += x * cr; acc[6] += y * cr; acc[7] += z * cr; acc[8] += l * cr;
+      return a;
+    };
+    if (!flush) {
+      // read-only stream, software-pipelined in registers: the loads of the
+      // next U rows are issued before the current U rows are consumed, so
+      // every wave keeps 2U rows (U x 24/40 bytes per lane) in flight
+      constexpr int U = QMCB_SM_UNROLL;
+      const double *__restrict__ ip = inv + c;
+      double ca[U], na[U];
+      TS cx[U], cy[U], cz[U], cl[U], nx[U], ny[U], nz[U], nl[U];
+      auto load = [&](int o0, double *a, TS *x, TS *y, TS *z, TS *l) {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          const int o = min(o0 + u, o_end - 1);  // clamped: in bounds, unused past o_end
+          const long off = (long)o * n;
+          a[u] = ld_stream(ip + off);
+          x[u] = ld_stream(d1 + off);
+          y[u] = ld_stream(d2 + off);
+          z[u] = ld_stream(d3 + off);
+          l[u] = ld_stream(d4 + off);
+        }
+      };
+      if (o_beg < o_end) load(o_beg, ca, cx, cy, cz, cl);
+      for (int o0 = o_beg; o0 < o_end; o0 += U) {
+        if (o0 + U < o_end) load(o0 + U, na, nx, ny, nz, nl);
+#pragma unroll
+        for (int u = 0; u < U; u++)
+          if (o0 + u < o_end)
+            row(o0 + u, ca[u], (double)cx[u], (double)cy[u], (double)cz[u], (double)cl[u]);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          ca[u] = na[u]; cx[u] = nx[u]; cy[u] = ny[u]; cz[u] = nz[u]; cl[u] = nl[u];
+        }
+      }
+    } else {
+      // the pending terms are folded into the stored inverse on the way
+      double *ip = inv + c;
+      for (int o = o_beg; o < o_end; o++) {
+        const long off = (long)o * n;
+        ip[off] = row(o, ld_stream(ip + off), (double)ld_stream(d1 + off),
+                      (double)ld_stream(d2 + off), (double)ld_stream(d3 + off),
+                      (double)ld_stream(d4 + off));
+      }
     }
   }
   if (sl > 0) {
