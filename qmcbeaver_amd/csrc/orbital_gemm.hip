@@ -8,7 +8,7 @@
 //   D^T[k][o][p] = sum_b  Ct[b][o] * X[k][b][p]
 //
 // Ct = C^T ([basis][orbital], zero padded to 16-multiples of orbitals and
-// 4-multiples of basis functions), X from kernel (a).  One wavefront owns a
+// 32-multiples of basis functions), X from kernel (a).  One wavefront owns a
 // 16-point tile and MT 16-orbital tiles; each k-step issues MT MFMA
 // 16x16x4 instructions (FP64: v_mfma_f64_16x16x4f64, FP32:
 // v_mfma_f32_16x16x4f32) with lane l supplying A[l%16][l/16] and
@@ -49,66 +49,82 @@ struct Mfma<float> {
   __device__ static constexpr int row(int lk, int i) { return 4 * lk + i; }
 };
 
+// Coefficient k-chunks are staged in LDS once per workgroup (4 waves share
+// them) and double-buffered: while the waves run the MFMAs of chunk c from
+// LDS, the next chunk is loaded from L2 into registers.  The point operand
+// X streams from HBM exactly once.  LDS rows are padded so that the four
+// k-rows one MFMA operand read touches (lanes 0-15 / 16-31 / ...) fall in
+// different banks.
+template <typename TI>
+struct GemmTile {
+  static constexpr int KC = 32;                       // k rows per chunk
+  static constexpr int PAD = sizeof(TI) == 4 ? 16 : 8;  // elements
+};
+
 template <typename TI, typename TO, int MT>
 __global__ void __launch_bounds__(256)
 orbital_gemm_kernel(const TI *__restrict__ Ct, int ldc, int n_orb, const TI *__restrict__ X,
                     int ldk, long ldp, long P, int kpad, GemmOut o) {
   typedef Mfma<TI> M;
   typedef typename M::acc_t acc_t;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  constexpr int KC = GemmTile<TI>::KC;
+  constexpr int OW = 16 * MT;                    // orbital columns of the tile
+  constexpr int LW = OW + GemmTile<TI>::PAD;     // LDS row length
+  constexpr int VE = 16 / sizeof(TI);            // elements per 16-byte load
+  constexpr int NV = (KC * OW + 256 * VE - 1) / (256 * VE);  // 16-byte loads per thread, chunk
+  static_assert(KC * OW % VE == 0, "tile / vector size mismatch");
+  __shared__ TI cs[2][KC][LW];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
   const long p0 = ((long)blockIdx.x * 4 + wave) * 16;
   const bool live = p0 < P;  // wave-uniform
   const int comp = blockIdx.y;
-  const int o0 = blockIdx.z * 16 * MT;
+  const int o0 = blockIdx.z * OW;
   const int li = lane & 15, lk = lane >> 4;
   acc_t acc[MT];
 #pragma unroll
   for (int t = 0; t < MT; t++) acc[t] = acc_t{0, 0, 0, 0};
-  if (live && sizeof(TI) == 8) {
-    // FP64: k advances in super-steps of 8 (two MFMA k-steps); the operands of
-    // the next super-step are loaded before the MFMAs of the current one, so
-    // the L2 latency of the coefficient tile overlaps the matrix-core work
-    // (kpad is a multiple of 8; padded rows of Ct and X are zero).  Measured
-    // 10% faster for FP64 and slower for FP32, whose MFMA steps are half as
-    // long and whose doubled register footprint costs occupancy.
-    const TI *__restrict__ xp = X + (long)comp * ldk * ldp + (long)lk * ldp + p0 + li;
-    const TI *__restrict__ cp = Ct + (long)lk * ldc + o0 + li;
-    TI a0[2][MT], b0[2], a1[2][MT], b1[2];
-    auto load = [&](int k0, TI (&a)[2][MT], TI (&b)[2]) {
+
+  // cooperative chunk load: element e = (j * 256 + tid) * VE of the KC x OW tile
+  typedef __attribute__((ext_vector_type(4))) unsigned int u4;
+  u4 stage[NV];
+  auto gload = [&](int k0) {
 #pragma unroll
-      for (int s = 0; s < 2; s++) {
-        const long k = k0 + 4 * s;
-        b[s] = xp[k * ldp];
+    for (int j = 0; j < NV; j++) {
+      const int e = (j * 256 + tid) * VE, r = e / OW, c = e % OW;
+      if (e < KC * OW) stage[j] = *(const u4 *)(Ct + (long)(k0 + r) * ldc + o0 + c);
+    }
+  };
+  auto lstore = [&](int buf) {
 #pragma unroll
-        for (int t = 0; t < MT; t++) a[s][t] = cp[k * ldc + 16 * t];
+    for (int j = 0; j < NV; j++) {
+      const int e = (j * 256 + tid) * VE, r = e / OW, c = e % OW;
+      if (e < KC * OW) *(u4 *)&cs[buf][r][c] = stage[j];
+    }
+  };
+  const TI *__restrict__ xp = X + (long)comp * ldk * ldp + (long)lk * ldp + p0 + li;
+  const int nch = kpad / KC;
+  gload(0);
+  lstore(0);
+  __syncthreads();
+  for (int ch = 0; ch < nch; ch++) {
+    const int buf = ch & 1, k0 = ch * KC;
+    if (ch + 1 < nch) gload(k0 + KC);
+    if (live) {
+      TI b[KC / 4];
+#pragma unroll
+      for (int s = 0; s < KC / 4; s++) b[s] = xp[(long)(k0 + 4 * s) * ldp];
+#pragma unroll
+      for (int s = 0; s < KC / 4; s++) {
+        TI a[MT];
+#pragma unroll
+        for (int t = 0; t < MT; t++) a[t] = cs[buf][4 * s + lk][16 * t + li];
+#pragma unroll
+        for (int t = 0; t < MT; t++) acc[t] = M::run(a[t], b[s], acc[t]);
       }
-    };
-    auto step = [&](const TI (&a)[2][MT], const TI (&b)[2]) {
-#pragma unroll
-      for (int s = 0; s < 2; s++)
-#pragma unroll
-        for (int t = 0; t < MT; t++) acc[t] = M::run(a[s][t], b[s], acc[t]);
-    };
-    load(0, a0, b0);
-    int k0 = 0;
-    for (; k0 + 16 <= kpad; k0 += 16) {
-      load(k0 + 8, a1, b1);
-      step(a0, b0);
-      if (k0 + 16 < kpad) load(k0 + 16, a0, b0);
-      step(a1, b1);
     }
-    if (k0 < kpad) step(a0, b0);  // kpad = 8 (mod 16): one super-step left
-  } else if (live) {
-    const TI *__restrict__ xp = X + (long)comp * ldk * ldp + (long)lk * ldp + p0 + li;
-    const TI *__restrict__ cp = Ct + (long)lk * ldc + o0 + li;
-    for (int k0 = 0; k0 < kpad; k0 += 4) {
-      const TI b = xp[(long)k0 * ldp];
-      TI a[MT];
-#pragma unroll
-      for (int t = 0; t < MT; t++) a[t] = cp[(long)k0 * ldc + 16 * t];
-#pragma unroll
-      for (int t = 0; t < MT; t++) acc[t] = M::run(a[t], b, acc[t]);
-    }
+    if (ch + 1 < nch) lstore(buf ^ 1);
+    __syncthreads();
   }
   TO *out = (TO *)o.out;
   if (o.s_orb == 1 && o.per_walker == 1 && !o.walker_map) {

@@ -450,7 +450,7 @@ qmcb_status create_impl(const qmcb_system *s, const qmcb_options *o, qmcb_ctx *c
   ctx->nn = s->n_nuc;
   ctx->B = s->n_basis;
   ctx->nprim = s->n_prim;
-  ctx->kpad = ((s->n_basis + 7) / 8) * 8;  // GEMM k super-steps of 8
+  ctx->kpad = ((s->n_basis + 31) / 32) * 32;  // GEMM k-chunks of 32 (zero padded)
   ctx->Wc = o->max_walkers;
   const bool f64 = o->precision == QMCB_FP64;
   const bool store32 = o->precision == QMCB_MIXED_F32STORE;
