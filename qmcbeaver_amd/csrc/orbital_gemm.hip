@@ -104,16 +104,25 @@ orbital_gemm_kernel(const TI *__restrict__ Ct, int ldc, int n_orb, const TI *__r
   };
   const TI *__restrict__ xp = X + (long)comp * ldk * ldp + (long)lk * ldp + p0 + li;
   const int nch = kpad / KC;
+  // the point operand of the next chunk is loaded with the next coefficient
+  // chunk, one chunk ahead of its MFMAs (its HBM latency overlaps them)
+  TI b[KC / 4], bn[KC / 4];
+  auto bload = [&](int k0, TI *dst) {
+    if (!live) return;
+#pragma unroll
+    for (int s = 0; s < KC / 4; s++) dst[s] = xp[(long)(k0 + 4 * s) * ldp];
+  };
   gload(0);
+  bload(0, b);
   lstore(0);
   __syncthreads();
   for (int ch = 0; ch < nch; ch++) {
     const int buf = ch & 1, k0 = ch * KC;
-    if (ch + 1 < nch) gload(k0 + KC);
+    if (ch + 1 < nch) {
+      gload(k0 + KC);
+      bload(k0 + KC, bn);
+    }
     if (live) {
-      TI b[KC / 4];
-#pragma unroll
-      for (int s = 0; s < KC / 4; s++) b[s] = xp[(long)(k0 + 4 * s) * ldp];
 #pragma unroll
       for (int s = 0; s < KC / 4; s++) {
         TI a[MT];
@@ -123,7 +132,11 @@ orbital_gemm_kernel(const TI *__restrict__ Ct, int ldc, int n_orb, const TI *__r
         for (int t = 0; t < MT; t++) acc[t] = M::run(a[t], b[s], acc[t]);
       }
     }
-    if (ch + 1 < nch) lstore(buf ^ 1);
+    if (ch + 1 < nch) {
+      lstore(buf ^ 1);
+#pragma unroll
+      for (int s = 0; s < KC / 4; s++) b[s] = bn[s];
+    }
     __syncthreads();
   }
   TO *out = (TO *)o.out;
