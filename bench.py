@@ -29,6 +29,21 @@ sys.path.insert(0, ROOT)
 
 METRIC = "walker-moves/s at 256 e⁻ (mixed vs FP64, same GPU) + p50/p99 block latency"
 HBM_PEAK_GBS = 8000.0
+PROFILE = os.path.join(ROOT, "profiles", "sm_move_mol256.md")
+
+
+def measured_traffic(precision):
+    """HBM bytes per sm_move launch of this workload, from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes summarised in profiles/sm_move_mol256.md
+    (scaled by the plain-copy calibration stated there); None if absent."""
+    try:
+        for line in open(PROFILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                kv = dict(t.split("=") for t in line.split(":", 1)[1].split())
+                return float(kv["fp64" if precision == "fp64" else "mixed"])
+    except (OSError, KeyError, ValueError):
+        pass
+    return None
 
 
 def sm_bytes_per_move(n, N, s_d, s_o, acc):
@@ -261,7 +276,11 @@ def main():
         "mean_e_local": mixed["mean_e_local"],
         "roofline": {"kernel": "sm_move", "bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "avg_launch_ms": avg_ms,
+                     "traffic": measured_traffic(args.precision)
+                     if args.walkers == 8192 and args.config == 2 else None,
+                     "traffic_source": "rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1 per dispatch, "
+                                       "profiles/sm_move_mol256.md",
+                     "avg_launch_ms": avg_ms,
                      "algorithmic_bytes_per_launch": bytes_launch},
         "kernels_ms_per_step": {k: v["ms"] / max(args.steps, 1) for k, v in mixed["kernels"].items()},
         "init_s": mixed["init_s"],
