@@ -86,6 +86,17 @@ typedef struct qmcb_system {
   qmcb_pade ee_updn, ee_upup, ee_dndn;
   const qmcb_pade *en_up;   /* [n_nuc] e-n function seen by alpha electrons    */
   const qmcb_pade *en_dn;   /* [n_nuc] e-n function seen by beta electrons     */
+  /* Several determinants, Psi = sum_d ci_coef[d] det_up(d) det_dn(d) e^U
+   * (QMCSCFJastrow.cpp:281-327).  n_det <= 1 with NULL tables: one
+   * determinant of the n_up / n_dn rows of mo_up / mo_dn.  Otherwise mo_up has
+   * n_act_up rows (the active orbitals of the alpha spin, in orbital order) and
+   * determinant d uses rows occ_up[d][0..n_up) as its columns
+   * (QMCWavefunction.cpp:873-905); likewise for beta. */
+  int32_t n_det;
+  int32_t n_act_up, n_act_dn;
+  const double *ci_coef;    /* [n_det]                                         */
+  const int32_t *occ_up;    /* [n_det][n_up]                                   */
+  const int32_t *occ_dn;    /* [n_det][n_dn]                                   */
 } qmcb_system;
 
 typedef struct qmcb_options {

@@ -34,6 +34,8 @@ class SystemStruct(C.Structure):
         ("mo_up", _dp), ("mo_dn", _dp),
         ("ee_updn", Pade), ("ee_upup", Pade), ("ee_dndn", Pade),
         ("en_up", C.POINTER(Pade)), ("en_dn", C.POINTER(Pade)),
+        ("n_det", C.c_int32), ("n_act_up", C.c_int32), ("n_act_dn", C.c_int32),
+        ("ci_coef", _dp), ("occ_up", _ip), ("occ_dn", _ip),
     ]
 
 
@@ -119,6 +121,18 @@ class SystemBuffers:
         self.en_dn = (Pade * n)(*[_pade_struct(r) for r in a["en_dn"]])
         s.en_up = C.cast(self.en_up, C.POINTER(Pade))
         s.en_dn = C.cast(self.en_dn, C.POINTER(Pade))
+        if int(packed.get("n_det", 1)) > 1 or "occ_up" in packed:
+            s.n_det = int(packed["n_det"])
+            s.n_act_up = int(packed["n_act_up"])
+            s.n_act_dn = int(packed["n_act_dn"])
+            for k in ("ci_coef",):
+                assert a[k].dtype == np.float64
+                setattr(s, k, a[k].ctypes.data_as(_dp))
+            for k in ("occ_up", "occ_dn"):
+                assert a[k].dtype == np.int32
+                setattr(s, k, a[k].ctypes.data_as(_ip))
+        else:
+            s.n_det = 1
         self.struct = s
         self.N = int(packed["n_up"]) + int(packed["n_dn"])
 

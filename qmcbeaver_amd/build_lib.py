@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libqmcb_amd.so")
-SOURCES = ["basis.hip", "orbital_gemm.hip", "slater.hip", "jastrow.hip", "cusp.hip",
+SOURCES = ["basis.hip", "orbital_gemm.hip", "slater.hip", "jastrow.hip", "cusp.hip", "multidet.hip",
            "qmcb_abi.cpp", "qmcb_stats.cpp", "qmcb_cusp.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

@@ -10,8 +10,10 @@ Semantics follow the reference parsers:
   * &wavefunction: restricted (or unrestricted: Alpha then Beta blocks with a
     three-token header each) orbital coefficients, `Alpha Occupation`,
     `Beta Occupation`, `CI Coeffs` (QMCWavefunction.cpp:311-472); the per-spin
-    coefficient matrices are the occupied rows in orbital order
-    (QMCWavefunction.cpp:873-905);
+    coefficient matrices are the rows of the orbitals any determinant
+    occupies, in orbital order, and each determinant lists its columns among
+    them (QMCWavefunction.cpp:873-905); several determinants add the packed
+    keys n_det, n_act_up/dn, ci_coef, occ_up/dn;
   * &Jastrow: two-body blocks (QMCCorrelationFunctionParameters.cpp:59-279)
     of type Pade (numerator parameters first, PadeCorrelationFunction.cpp:15-51)
     or FixedCuspPade (constants then parameters in numerator and denominator,
@@ -19,8 +21,8 @@ Semantics follow the reference parsers:
     copies up-up to down-down and up-nucleus to down-nucleus
     (QMCJastrowParameters.cpp:880-900).
 
-Only single-determinant, all-electron inputs with Cartesian s/p/d functions
-and Pade-type two-body Jastrows map onto the kernels; anything else raises
+Only all-electron inputs with Cartesian s/p/d functions and Pade-type
+two-body Jastrows map onto the kernels; anything else raises
 `UnsupportedInput` naming the feature.
 """
 from __future__ import annotations
@@ -117,8 +119,6 @@ def read_ckmf(path: str):
     link = int(flags.get("link_Jastrow_parameters", "1"))
     if int(flags.get("use_three_body_jastrow", "0")):
         raise UnsupportedInput("three-body Jastrow")
-    if ndet != 1:
-        raise UnsupportedInput(f"{ndet} determinants (single determinant only)")
     # ---- geometry
     g = _section(tok, "geometry")
     labels, Z, xyz = [], [], []
@@ -168,21 +168,50 @@ def read_ckmf(path: str):
         orbs = np.vstack([oa, ob])
     else:
         raise UnsupportedInput(f"trial_function_type {tft}")
+    # occupations: ndet rows of 0/1 per spin; the beta rows of an unrestricted
+    # input index the second orbital block (QMCWavefunction.cpp:364-381)
     pos += 2  # "Alpha Occupation"
-    occ_a = [int(v) for v in w[pos:pos + norb]]; pos += norb
+    occ_a = np.array([int(v) for v in w[pos:pos + ndet * norb]]).reshape(ndet, norb)
+    pos += ndet * norb
     pos += 2  # "Beta Occupation"
-    occ_b = [int(v) for v in w[pos:pos + norb]]; pos += norb
+    occ_b = np.array([int(v) for v in w[pos:pos + ndet * norb]]).reshape(ndet, norb)
+    pos += ndet * norb
     pos += 2  # "CI Coeffs"
-    ci = float(w[pos])
-    alpha_rows = [i for i in range(norb) if occ_a[i]]
-    beta_rows = [i + (nrows - norb) for i in range(norb) if occ_b[i]]
-    n_up, n_dn = len(alpha_rows), len(beta_rows)
+    ci = np.zeros(ndet)
+    constraint = [-1] * ndet
+    factor = [1.0] * ndet
+    for d in range(ndet):
+        t = w[pos]; pos += 1
+        if "c" in t:  # "c <det> <factor>": tied to another coefficient (:386-409)
+            c = int(w[pos]); f = float(w[pos + 1]); pos += 2
+            constraint[d] = c if constraint[c] == -1 else constraint[c]
+            factor[d] = f
+        else:
+            ci[d] = float(t)
+    for d in range(ndet):
+        if constraint[d] != -1:
+            ci[d] = factor[d] * ci[constraint[d]]
+    # per spin: active orbitals = used by any determinant, in orbital order; a
+    # determinant's columns are its occupied orbitals in increasing order
+    # (QMCWavefunction::makeCoefficients, QMCWavefunction.cpp:873-905)
+    def spin_tables(occ, row_offset):
+        act = [o for o in range(norb) if occ[:, o].any()]
+        index = {o: i for i, o in enumerate(act)}
+        cols = [[index[o] for o in range(norb) if occ[d, o]] for d in range(ndet)]
+        return act, cols
+    act_a, cols_a = spin_tables(occ_a, 0)
+    act_b, cols_b = spin_tables(occ_b, nrows - norb)
+    n_up, n_dn = len(cols_a[0]), len(cols_b[0])
+    if any(len(c) != n_up for c in cols_a) or any(len(c) != n_dn for c in cols_b):
+        raise UnsupportedInput("determinants with different electron counts")
     if n_dn > n_up:
         raise UnsupportedInput("more down than up electrons")
     if sum(Z) != n_up + n_dn + charge:
         raise UnsupportedInput("nuclear charge does not match electrons + charge")
-    mo_up = np.ascontiguousarray(orbs[alpha_rows])
-    mo_dn = np.ascontiguousarray(orbs[beta_rows]) if n_dn else np.zeros((0, nbf))
+    mo_up = np.ascontiguousarray(orbs[act_a])
+    mo_dn = (np.ascontiguousarray(orbs[[o + (nrows - norb) for o in act_b]]) if n_dn
+             else np.zeros((0, nbf)))
+    multi = ndet > 1 or cols_a[0] != list(range(n_up)) or cols_b[0] != list(range(n_dn))
     # ---- Jastrow
     j = _jastrow_tokens(tok)
     funcs = _parse_jastrow(j)
@@ -210,6 +239,12 @@ def read_ckmf(path: str):
         "ee_dndn": _pade_arr(ee_dndn),
         "en_up": np.stack(en_up), "en_dn": np.stack(en_dn),
     }
+    if multi:
+        packed.update({
+            "n_det": ndet, "n_act_up": len(act_a), "n_act_dn": len(act_b),
+            "ci_coef": np.asarray(ci, dtype=np.float64),
+            "occ_up": np.asarray(cols_a, dtype=np.int32).reshape(ndet, n_up),
+            "occ_dn": np.asarray(cols_b, dtype=np.int32).reshape(ndet, n_dn)})
     info = {"labels": labels, "ci_coeff": ci,
             "cusp_replacement": int(flags.get("replace_electron_nucleus_cusps", "1"))}
     return packed, flags, info

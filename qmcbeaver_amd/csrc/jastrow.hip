@@ -191,31 +191,6 @@ hipError_t launch_jastrow_move(const JastrowArgs &a, const double *U, const doub
   return hipGetLastError();
 }
 
-// ---------------------------------------------------------------------------
-// Drift modification helper shared with the move kernel (defined in slater.hip)
-__device__ inline double drift_factor_unequal_j(const double g[3], const double x[3], int nn,
-                                                const double *nuc_xyz, const double *nuc_z,
-                                                double dt) {
-  int cn = 0;
-  double best = 1e300;
-  for (int A = 0; A < nn; A++) {
-    const double dx = x[0] - nuc_xyz[3 * A], dy = x[1] - nuc_xyz[3 * A + 1],
-                 dz = x[2] - nuc_xyz[3 * A + 2];
-    const double r = sqrt(dx * dx + dy * dy + dz * dz);
-    if (r < best) { best = r; cn = A; }
-  }
-  double nv[3], dsq = 0.0;
-  for (int k = 0; k < 3; k++) { nv[k] = x[k] - nuc_xyz[3 * cn + k]; dsq += nv[k] * nv[k]; }
-  const double inr = 1.0 / sqrt(dsq);
-  const double v2 = g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
-  if (fabs(v2) < 1e-10) return 1.0;
-  const double iv = 1.0 / sqrt(v2);
-  const double dotp = (nv[0] * g[0] + nv[1] * g[1] + nv[2] * g[2]) * inr * iv;
-  const double Z = nuc_z[cn];
-  const double t = Z * Z * dsq;
-  const double a = 0.5 * (1.0 + dotp) + t / (10.0 * (4.0 + t));
-  return (-1.0 + sqrt(1.0 + 2.0 * a * v2 * dt)) / (a * v2 * dt);
-}
 
 // all-electron assembly, one workgroup per walker
 __global__ void __launch_bounds__(kJT) assemble_kernel(AssembleArgs A, const int *walkers, int num) {
@@ -269,7 +244,7 @@ __global__ void __launch_bounds__(kJT) assemble_kernel(AssembleArgs A, const int
       double f = feq;
       if (A.drift_mod == QMCB_DRIFT_UMRIGAR93_UNEQUAL) {
         const double x[3] = {Rw[3 * e], Rw[3 * e + 1], Rw[3 * e + 2]};
-        f = drift_factor_unequal_j(g, x, S.nn, S.nuc_xyz, S.nuc_z, A.dt);
+        f = drift_factor_unequal(g, x, S.nn, S.nuc_xyz, S.nuc_z, A.dt);
       }
       double *o = A.out.mod_drift + ((long)b * N + e) * 3;
       o[0] = g[0] * f; o[1] = g[1] * f; o[2] = g[2] * f;
@@ -313,7 +288,8 @@ __global__ void __launch_bounds__(256) propose_kernel(ProposeArgs A, const int *
   const int sp = m < A.n_up ? 0 : 1;
   const int le = sp == 0 ? m : m - A.n_up;
   const int nsp = sp == 0 ? A.n_up : A.N - A.n_up;
-  const double *gp = A.gradPR[sp] + ((long)w * nsp + le) * 3;
+  const double *gp = A.cgrad ? A.cgrad + ((long)w * A.N + m) * 3
+                             : A.gradPR[sp] + ((long)w * nsp + le) * 3;
   const double *ux = A.Ux + ((long)w * A.N + m) * 3;
   const double *x = A.R + ((long)w * A.N + m) * 3;
   const uint32_t k0 = (uint32_t)A.seed, k1 = (uint32_t)(A.seed >> 32);

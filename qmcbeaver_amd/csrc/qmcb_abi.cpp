@@ -81,6 +81,16 @@ struct qmcb_ctx {
   uint8_t *d_mask = nullptr;
   // vmc accumulators
   double *acc_el = nullptr, *acc_el2 = nullptr, *acc_n = nullptr, *acc_prop_el = nullptr;
+  // several determinants: per-determinant records (inverse, log-det, traces)
+  // at w * ndet + d; active orbitals nact[sp] >= n[sp] share one GEMM
+  int ndet = 1;
+  bool multi = false;
+  int nact[2] = {0, 0};
+  double *d_ci = nullptr;
+  int *d_occ[2] = {nullptr, nullptr};
+  int *st_rowsing = nullptr;  // [Wc] combined proposal failed
+  double *cgrad = nullptr;     // [Wc][N][3] grad ln D of the determinant sum
+  double *st_cgrad = nullptr;  // [Wc][N][3] the same at a staged proposal
   // electron-nucleus cusp replacement (per spin)
   bool cusp = false;
   CuspDev cuspd[2] = {};
@@ -226,6 +236,48 @@ SlaterArgs slater_args(qmcb_ctx *ctx, int sp) {
   a.dcount = ctx->dcount[sp];
   a.ubuf = ctx->ubuf[sp];
   a.zbuf = ctx->zbuf[sp];
+  a.ndet = ctx->ndet;
+  a.nact = ctx->nact[sp];
+  a.occ = ctx->multi ? ctx->d_occ[sp] : nullptr;
+  return a;
+}
+
+// several determinants: arguments of the staging / assembly / commit kernels
+MdetArgs mdet_args(qmcb_ctx *ctx, int m, int mode, const qmcb_results &out) {
+  MdetArgs a{};
+  a.s[0] = slater_args(ctx, 0);
+  a.s[1] = slater_args(ctx, 1);
+  a.D = ctx->ndet;
+  a.N = ctx->N;
+  a.moved = m;
+  a.ci = ctx->d_ci;
+  const int sp = (m >= 0 && m >= ctx->n[0]) ? 1 : 0;
+  a.vrow = ctx->vrow;
+  a.v_s_comp = ctx->npad[sp];
+  a.v_s_walker = (long)kComp * ctx->npad[sp];
+  a.st_inv = ctx->st_inv; a.st_grad = ctx->st_grad; a.st_lap = ctx->st_lap;
+  a.st_logdet = ctx->st_logdet; a.st_sgn = ctx->st_sgn; a.st_singular = ctx->st_singular;
+  a.st_row_singular = ctx->st_rowsing;
+  a.R = ctx->R;
+  a.propR = ctx->propR;
+  a.U = ctx->U; a.Ux = ctx->Ux; a.Uxx = ctx->Uxx; a.Vee = ctx->Vee; a.Ven = ctx->Ven;
+  a.pU = ctx->pU; a.pUx = ctx->pUx; a.pUxx = ctx->pUxx; a.pVee = ctx->pVee; a.pVen = ctx->pVen;
+  a.v_nn = ctx->dsys.v_nn;
+  a.n_nuc = ctx->nn;
+  a.nuc_xyz = ctx->dsys.nuc_xyz;
+  a.nuc_z = ctx->dsys.nuc_z;
+  a.drift_mod = ctx->opt.drift_mod;
+  a.dt = ctx->opt.dt;
+  a.umrigar_a = ctx->opt.umrigar_a;
+  a.out = out;
+  a.mode = mode;
+  a.gauss = ctx->gauss;
+  a.unif = ctx->unif;
+  a.mask = ctx->d_mask;
+  a.acc_el = ctx->acc_el; a.acc_el2 = ctx->acc_el2; a.acc_n = ctx->acc_n;
+  a.acc_prop_el = ctx->acc_prop_el; a.cur_el = ctx->cur_el;
+  a.cgrad = ctx->cgrad;
+  a.st_cgrad = ctx->st_cgrad;
   return a;
 }
 
@@ -249,9 +301,11 @@ struct Engine {
                      d_map + c0};
         TIMED("basis", (launch_basis<TC, TX>(ctx->dsys, pts, (TX *)ctx->X, ctx->kpad, ctx->ldp,
                                             ctx->stream)));
-        GemmOut go{ctx->dmat[sp], (long)n * n, (long)n, (long)kDer * n * n, 1, n, d_map + c0,
-                   ctx->dval[sp], (long)n * n};
-        TIMED("orbital_gemm", (launch_orbital_gemm<TX, TS>((const TX *)ctx->Ct[sp], ctx->npad[sp], n,
+        const long na = ctx->nact[sp];
+        GemmOut go{ctx->dmat[sp], na * n, (long)n, (long)kDer * na * n, 1, n, d_map + c0,
+                   ctx->dval[sp], na * n};
+        TIMED("orbital_gemm", (launch_orbital_gemm<TX, TS>((const TX *)ctx->Ct[sp], ctx->npad[sp],
+                                                          (int)na,
                                                           (const TX *)ctx->X, ctx->kpad, ctx->ldp,
                                                           (long)cw * n, ctx->kpad, go,
                                                           ctx->stream)));
@@ -271,6 +325,11 @@ struct Engine {
   }
 
   static qmcb_status assemble(qmcb_ctx *ctx, const int *d_map, int num, const qmcb_results &out) {
+    if (ctx->multi) {
+      MdetArgs ma = mdet_args(ctx, -1, 0, out);
+      TIMED("mdet_assemble", launch_mdet_assemble(ma, d_map, num, ctx->stream));
+      return QMCB_OK;
+    }
     AssembleArgs aa{};
     aa.sys = ctx->dsys;
     aa.s[0] = slater_args(ctx, 0);
@@ -303,9 +362,11 @@ struct Engine {
     const int n = ctx->n[sp];
     GemmOut go{ctx->vrow, (long)ctx->npad[sp], 1, (long)kComp * ctx->npad[sp], 0, 1, nullptr,
                nullptr, 0};
-    TIMED("orbital_gemm", (launch_orbital_gemm<TX, TO>((const TX *)ctx->Ct[sp], ctx->npad[sp], n,
-                                                      (const TX *)ctx->X, ctx->kpad, ctx->ldp,
-                                                      (long)num, ctx->kpad, go, ctx->stream)));
+    TIMED("orbital_gemm", (launch_orbital_gemm<TX, TO>((const TX *)ctx->Ct[sp], ctx->npad[sp],
+                                                      ctx->nact[sp], (const TX *)ctx->X,
+                                                      ctx->kpad, ctx->ldp, (long)num, ctx->kpad,
+                                                      go, ctx->stream)));
+    (void)n;
     if (ctx->cusp) TIMED("cusp", (launch_cusp<TO>(ctx->cuspd[sp], pts, go, ctx->stream)));
     return QMCB_OK;
   }
@@ -352,6 +413,25 @@ struct Engine {
   static qmcb_status commit(qmcb_ctx *ctx, const MoveArgs &a, const int *d_map,
                             const uint8_t *d_mask, int num) {
     TIMED("commit", (launch_commit<TS, TO>(a, d_map, d_mask, num, ctx->stream)));
+    return QMCB_OK;
+  }
+
+  // several determinants: stage the proposal of every determinant, combine
+  // (and, in VMC mode, accept on the device), commit accepted rows
+  static qmcb_status mdet_stage(qmcb_ctx *ctx, const int *d_map, int num, int m, int mode,
+                                const qmcb_results &out, double tau, double *obs) {
+    MdetArgs ma = mdet_args(ctx, m, mode, out);
+    ma.tau = tau;
+    ma.obs = obs;
+    TIMED("mdet_move", (launch_mdet_move<TS, TO>(ma, d_map, num, ctx->stream)));
+    TIMED("mdet_assemble", launch_mdet_assemble(ma, d_map, num, ctx->stream));
+    return QMCB_OK;
+  }
+
+  static qmcb_status mdet_commit(qmcb_ctx *ctx, const int *d_map, const uint8_t *d_mask, int num,
+                                 int m) {
+    MdetArgs ma = mdet_args(ctx, m, 0, qmcb_results{});
+    TIMED("mdet_commit", (launch_mdet_commit<TS, TO>(ma, d_map, d_mask, num, ctx->stream)));
     return QMCB_OK;
   }
 
@@ -516,9 +596,45 @@ qmcb_status create_impl(const qmcb_system *s, const qmcb_options *o, qmcb_ctx *c
     }
   d.v_nn = vnn;
 
-  // ---- transposed, padded MO coefficients in the operand type
+  // ---- determinants: active orbitals and occupation maps
+  ctx->multi = s->n_det > 1 || s->occ_up || s->occ_dn;
+  ctx->ndet = std::max(1, (int)s->n_det);
+  if (ctx->ndet > 64) {
+    ctx->err = "at most 64 determinants";
+    return QMCB_INVALID_ARG;
+  }
   for (int sp = 0; sp < 2; sp++) {
     const int n = ctx->n[sp];
+    const int na = sp == 0 ? s->n_act_up : s->n_act_dn;
+    ctx->nact[sp] = (ctx->multi && na > 0) ? na : n;
+    if (ctx->nact[sp] < n) {
+      ctx->err = "fewer active orbitals than electrons";
+      return QMCB_INVALID_ARG;
+    }
+    if (!ctx->multi || n == 0) continue;
+    const int32_t *occ = sp == 0 ? s->occ_up : s->occ_dn;
+    if (!occ || !s->ci_coef) {
+      ctx->err = "several determinants need ci_coef, occ_up and occ_dn";
+      return QMCB_INVALID_ARG;
+    }
+    for (long i = 0; i < (long)ctx->ndet * n; i++)
+      if (occ[i] < 0 || occ[i] >= ctx->nact[sp]) {
+        ctx->err = "occupation index outside the active orbitals";
+        return QMCB_INVALID_ARG;
+      }
+    QCHK(upload(ctx, (const int **)&ctx->d_occ[sp], occ, (size_t)ctx->ndet * n));
+  }
+  if (ctx->multi) {
+    if (std::max(ctx->n[0], ctx->n[1]) > 128) {
+      ctx->err = "several determinants: at most 128 electrons per spin";
+      return QMCB_INVALID_ARG;
+    }
+    QCHK(upload(ctx, (const double **)&ctx->d_ci, s->ci_coef, (size_t)ctx->ndet));
+  }
+
+  // ---- transposed, padded MO coefficients (active orbitals) in the operand type
+  for (int sp = 0; sp < 2; sp++) {
+    const int n = ctx->nact[sp];
     ctx->npad[sp] = pad_orb(std::max(n, 1));
     const double *C = sp == 0 ? s->mo_up : s->mo_dn;
     const size_t cnt = (size_t)ctx->kpad * ctx->npad[sp];
@@ -545,11 +661,11 @@ qmcb_status create_impl(const qmcb_system *s, const qmcb_options *o, qmcb_ctx *c
       if (n == 0) continue;
       CuspTable t;
       std::string err;
-      if (!fit_cusps(s, sp == 0 ? s->mo_up : s->mo_dn, n, t, err)) {
+      if (!fit_cusps(s, sp == 0 ? s->mo_up : s->mo_dn, ctx->nact[sp], t, err)) {
         ctx->err = err;
         return QMCB_INVALID_ARG;
       }
-      std::vector<double> orb8((size_t)t.nn * n * 8);
+      std::vector<double> orb8((size_t)t.nn * ctx->nact[sp] * 8);
       for (size_t i = 0; i < t.orb.size(); i++) {
         const qmcb_cusp_orbital &q = t.orb[i];
         double *f = &orb8[i * 8];
@@ -558,7 +674,7 @@ qmcb_status create_impl(const qmcb_system *s, const qmcb_options *o, qmcb_ctx *c
       }
       CuspDev &c = ctx->cuspd[sp];
       c.nn = t.nn;
-      c.norb = n;
+      c.norb = ctx->nact[sp];
       c.ns = (int)t.s_alpha.size();
       c.nuc_xyz = d.nuc_xyz;
       QCHK(upload(ctx, &c.rc_max, t.rc_max.data(), t.rc_max.size()));
@@ -572,22 +688,24 @@ qmcb_status create_impl(const qmcb_system *s, const qmcb_options *o, qmcb_ctx *c
   // ---- resident walker state
   const long W = ctx->Wc, N = ctx->N;
   QCHK(dalloc(ctx, &ctx->R, W * N * 3));
+  const long WD = W * ctx->ndet;  // per-determinant records
   for (int sp = 0; sp < 2; sp++) {
-    const long n = ctx->n[sp];
-    QCHK(dalloc(ctx, &ctx->inv[sp], W * n * n));
-    QCHK(dalloc(ctx, &ctx->dval[sp], W * n * n));
-    QCHK(dalloc_bytes(ctx, &ctx->dmat[sp], (size_t)W * kDer * n * n * ctx->ts));
-    QCHK(dalloc(ctx, &ctx->logdet[sp], W));
-    QCHK(dalloc(ctx, &ctx->sgn[sp], W));
-    QCHK(dalloc(ctx, &ctx->gradPR[sp], W * std::max(n, 1L) * 3));
-    QCHK(dalloc(ctx, &ctx->lapPR[sp], W));
-    QCHK(dalloc(ctx, &ctx->singular[sp], W));
-    QCHK(dalloc(ctx, &ctx->dcount[sp], W));
+    const long n = ctx->n[sp], na = ctx->nact[sp];
+    QCHK(dalloc(ctx, &ctx->inv[sp], WD * n * n));
+    QCHK(dalloc(ctx, &ctx->dval[sp], W * na * n));
+    QCHK(dalloc_bytes(ctx, &ctx->dmat[sp], (size_t)W * kDer * na * n * ctx->ts));
+    QCHK(dalloc(ctx, &ctx->logdet[sp], WD));
+    QCHK(dalloc(ctx, &ctx->sgn[sp], WD));
+    QCHK(dalloc(ctx, &ctx->gradPR[sp], WD * std::max(n, 1L) * 3));
+    QCHK(dalloc(ctx, &ctx->lapPR[sp], WD));
+    QCHK(dalloc(ctx, &ctx->singular[sp], WD));
+    QCHK(dalloc(ctx, &ctx->dcount[sp], WD));
     QCHK(dalloc(ctx, &ctx->ubuf[sp], W * kDelay * std::max(n, 1L)));
     QCHK(dalloc(ctx, &ctx->zbuf[sp], W * kDelay * std::max(n, 1L)));
   }
   QCHK(dalloc(ctx, &ctx->U, W));
   QCHK(dalloc(ctx, &ctx->Ux, W * N * 3));
+  if (ctx->multi) QCHK(dalloc(ctx, &ctx->cgrad, W * N * 3));
   QCHK(dalloc(ctx, &ctx->Uxx, W));
   QCHK(dalloc(ctx, &ctx->Vee, W));
   QCHK(dalloc(ctx, &ctx->Ven, W));
@@ -644,7 +762,7 @@ qmcb_status create_impl(const qmcb_system *s, const qmcb_options *o, qmcb_ctx *c
 
 qmcb_status ensure_staging(qmcb_ctx *ctx) {
   if (ctx->st_inv) return QMCB_OK;
-  const long W = ctx->Wc;
+  const long W = ctx->Wc * ctx->ndet;  // one record per (batch row, determinant)
   const long nm = std::max(ctx->n[0], ctx->n[1]);
   QCHK(dalloc(ctx, &ctx->st_inv, W * nm * nm));
   QCHK(dalloc(ctx, &ctx->st_grad, W * nm * 3));
@@ -652,6 +770,8 @@ qmcb_status ensure_staging(qmcb_ctx *ctx) {
   QCHK(dalloc(ctx, &ctx->st_logdet, W));
   QCHK(dalloc(ctx, &ctx->st_sgn, W));
   QCHK(dalloc(ctx, &ctx->st_singular, W));
+  QCHK(dalloc(ctx, &ctx->st_rowsing, ctx->Wc));
+  if (ctx->multi) QCHK(dalloc(ctx, &ctx->st_cgrad, (long)ctx->Wc * ctx->N * 3));
   QCHK(hipStreamSynchronize(ctx->stream));
   return QMCB_OK;
 }
@@ -696,6 +816,7 @@ qmcb_status evaluate_core(qmcb_ctx *ctx, int num, const int *d_ids, const double
   if (s != QMCB_OK) return s;
   s = DISPATCH(ctx, move_front(ctx, d_ids, num, whichE));
   if (s != QMCB_OK) return s;
+  if (ctx->multi) return DISPATCH(ctx, mdet_stage(ctx, d_ids, num, whichE, 0, dout, 0.0, nullptr));
   MoveArgs a = DISPATCH(ctx, move_args(ctx, whichE, 0, dout));
   return DISPATCH(ctx, sm(ctx, a, d_ids, num));
 }
@@ -842,8 +963,14 @@ qmcb_status qmcb_accept(qmcb_ctx *ctx, int32_t num, const int32_t *ids, const ui
                       hipMemcpyHostToDevice, ctx->stream));
   QCHK(hipMemcpyAsync(ctx->d_mask, rowmask.data(), ctx->staged_num, hipMemcpyHostToDevice,
                       ctx->stream));
-  MoveArgs a = DISPATCH(ctx, move_args(ctx, ctx->staged_moved, 0, ctx->dout));
-  qmcb_status s = DISPATCH(ctx, commit(ctx, a, ctx->d_ids, ctx->d_mask, ctx->staged_num));
+  qmcb_status s;
+  if (ctx->multi) {
+    s = DISPATCH(ctx, mdet_commit(ctx, ctx->d_ids, ctx->d_mask, ctx->staged_num,
+                                  ctx->staged_moved));
+  } else {
+    MoveArgs a = DISPATCH(ctx, move_args(ctx, ctx->staged_moved, 0, ctx->dout));
+    s = DISPATCH(ctx, commit(ctx, a, ctx->d_ids, ctx->d_mask, ctx->staged_num));
+  }
   if (s != QMCB_OK) return s;
   QCHK(hipStreamSynchronize(ctx->stream));
   ctx->staged_moved = -1;
@@ -912,6 +1039,10 @@ qmcb_status qmcb_vmc_steps(qmcb_ctx *ctx, int32_t num, int64_t first_step, int32
   {
     qmcb_status rs = series_reserve(ctx, first_step, n_steps);
     if (rs != QMCB_OK) return rs;
+    if (ctx->multi) {
+      rs = ensure_staging(ctx);
+      if (rs != QMCB_OK) return rs;
+    }
   }
   qmcb_results none{};
   for (int s = 0; s < n_steps; s++) {
@@ -925,6 +1056,7 @@ qmcb_status qmcb_vmc_steps(qmcb_ctx *ctx, int32_t num, int64_t first_step, int32
     pa.gradPR[0] = ctx->gradPR[0];
     pa.gradPR[1] = ctx->gradPR[1];
     pa.Ux = ctx->Ux;
+    pa.cgrad = ctx->multi ? ctx->cgrad : nullptr;
     pa.tau = tau;
     pa.seed = ctx->opt.seed;
     pa.step = step;
@@ -934,10 +1066,19 @@ qmcb_status qmcb_vmc_steps(qmcb_ctx *ctx, int32_t num, int64_t first_step, int32
     TIMED("propose", launch_propose(pa, nullptr, num, ctx->stream));
     qmcb_status st = DISPATCH(ctx, move_front(ctx, nullptr, num, m));
     if (st != QMCB_OK) return st;
-    MoveArgs a = DISPATCH(ctx, move_args(ctx, m, 1, none));
-    a.tau = tau;
-    if (m == 0) a.obs = ctx->d_obs;
-    st = DISPATCH(ctx, sm(ctx, a, nullptr, num));
+    if (ctx->multi) {
+      // several determinants: stage every determinant, combine and accept on
+      // the device, then commit the accepted rows
+      st = DISPATCH(ctx, mdet_stage(ctx, nullptr, num, m, 1, none, tau,
+                                    m == 0 ? ctx->d_obs : nullptr));
+      if (st != QMCB_OK) return st;
+      st = DISPATCH(ctx, mdet_commit(ctx, nullptr, ctx->d_mask, num, m));
+    } else {
+      MoveArgs a = DISPATCH(ctx, move_args(ctx, m, 1, none));
+      a.tau = tau;
+      if (m == 0) a.obs = ctx->d_obs;
+      st = DISPATCH(ctx, sm(ctx, a, nullptr, num));
+    }
     if (st != QMCB_OK) return st;
     if (m == 0)
       TIMED("series", launch_series_reduce(ctx->d_obs, num, ctx->d_series + 4L * ctx->series_len++,
@@ -1008,6 +1149,10 @@ static CopyArgs walker_copy_args(qmcb_ctx *ctx) {
 
 qmcb_status qmcb_dmc_init(qmcb_ctx *ctx, int32_t num) {
   if (!ctx || num < 1 || num > ctx->Wc) return QMCB_INVALID_ARG;
+  if (ctx->multi) {
+    ctx->err = "device DMC supports one determinant";
+    return QMCB_INVALID_ARG;
+  }
   for (int i = 0; i < num; i++)
     if (!ctx->initialized[i]) {
       ctx->err = "dmc slots must be initialised by an all-electron evaluate";
@@ -1078,6 +1223,7 @@ qmcb_status qmcb_dmc_steps(qmcb_ctx *ctx, int64_t first_step, int32_t n_steps,
     pa.gradPR[0] = ctx->gradPR[0];
     pa.gradPR[1] = ctx->gradPR[1];
     pa.Ux = ctx->Ux;
+    pa.cgrad = ctx->multi ? ctx->cgrad : nullptr;
     pa.tau = prm->tau;
     pa.seed = ctx->opt.seed;
     pa.step = step;

@@ -69,6 +69,32 @@ __device__ __forceinline__ void pade_eval(const qmcb_pade &f, double r, double &
 }
 
 // ---- wavefront reductions (64 lanes) ---------------------------------------
+// umrigar93_unequalelectrons drift factor of one electron with gradient g at
+// x: nearest nucleus, its charge and the time step (QMCSCFJastrow.cpp:368-443).
+__device__ inline double drift_factor_unequal(const double g[3], const double x[3], int nn,
+                                              const double *nuc_xyz, const double *nuc_z,
+                                              double dt) {
+  int cn = 0;
+  double best = 1e300;
+  for (int A = 0; A < nn; A++) {
+    const double dx = x[0] - nuc_xyz[3 * A], dy = x[1] - nuc_xyz[3 * A + 1],
+                 dz = x[2] - nuc_xyz[3 * A + 2];
+    const double r = sqrt(dx * dx + dy * dy + dz * dz);
+    if (r < best) { best = r; cn = A; }
+  }
+  double nv[3], dsq = 0.0;
+  for (int k = 0; k < 3; k++) { nv[k] = x[k] - nuc_xyz[3 * cn + k]; dsq += nv[k] * nv[k]; }
+  const double inr = 1.0 / sqrt(dsq);
+  const double v2 = g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
+  if (fabs(v2) < 1e-10) return 1.0;
+  const double iv = 1.0 / sqrt(v2);
+  const double dotp = (nv[0] * g[0] + nv[1] * g[1] + nv[2] * g[2]) * inr * iv;
+  const double Z = nuc_z[cn];
+  const double t = Z * Z * dsq;
+  const double a = 0.5 * (1.0 + dotp) + t / (10.0 * (4.0 + t));
+  return (-1.0 + sqrt(1.0 + 2.0 * a * v2 * dt)) / (a * v2 * dt);
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);

@@ -72,7 +72,19 @@ struct SlaterArgs {
   //   A^-1 = inv - sum_{a < dcount} ubuf[a] zbuf[a]^T
   int *dcount;           // [Wc]
   double *ubuf, *zbuf;   // [Wc][kDelay][n]
+  // several determinants: the per-determinant arrays above (inv, logdet, sgn,
+  // gradPR, lapPR, singular, dcount) are indexed by w * ndet + d, while dval /
+  // dmat hold all nact active orbitals of walker w ([Wc][nact][n],
+  // [Wc][4][nact][n]); determinant d takes its column c from active orbital
+  // occ[d * n + c].  ndet <= 1, occ = null: one determinant, nact = n.
+  int ndet, nact;
+  const int *occ;
 };
+__device__ __forceinline__ int det_count(const SlaterArgs &A) { return A.ndet > 1 ? A.ndet : 1; }
+__device__ __forceinline__ int act_count(const SlaterArgs &A) { return A.nact > 0 ? A.nact : A.n; }
+__device__ __forceinline__ int occ_row(const SlaterArgs &A, int d, int c) {
+  return A.occ ? A.occ[d * A.n + c] : c;
+}
 constexpr int kDelay = 8;
 constexpr int kDer = 4;  // stored derivative components
 
@@ -124,6 +136,45 @@ struct MoveArgs {
 // series from obs[3 * num] (single workgroup, fixed summation order).
 hipError_t launch_series_reduce(const double *obs, int num, double *row, hipStream_t st);
 
+// several determinants (multidet.hip): per-spin Slater records with ndet /
+// nact / occ set, staging per (batch row, determinant) at b * D + d
+struct MdetArgs {
+  SlaterArgs s[2];
+  int D;                 // determinants
+  int N;
+  int moved;             // moved electron (global index); -1: all-electron assembly
+  const double *ci;      // [D]
+  const void *vrow;      // new orbital rows of the moved electron (GEMM output type)
+  long v_s_comp, v_s_walker;
+  double *st_inv, *st_grad, *st_lap, *st_logdet, *st_sgn;  // [num * D] records
+  int *st_singular;      // [num * D]
+  int *st_row_singular;  // [num]: the combined proposal failed
+  double *R;
+  const double *propR;
+  double *U, *Ux, *Uxx, *Vee, *Ven;
+  const double *pU, *pUx, *pUxx, *pVee, *pVen;
+  double v_nn;
+  int n_nuc;
+  const double *nuc_xyz, *nuc_z;
+  int drift_mod;
+  double dt, umrigar_a;
+  qmcb_results out;      // device outputs per batch row
+  int mode;              // 0: evaluate / stage, 1: VMC Metropolis on the device
+  double tau;
+  const double *gauss, *unif;
+  uint8_t *mask;         // [num] accept decisions (VMC)
+  double *acc_el, *acc_el2, *acc_n, *acc_prop_el, *cur_el;
+  double *obs;           // per-sweep observable rows (VMC, electron 0 moves)
+  double *cgrad;         // [Wc][N][3] grad ln D of the determinant sum (resident)
+  double *st_cgrad;      // [num][N][3] the same at the proposal
+};
+template <typename TS, typename TO>
+hipError_t launch_mdet_move(const MdetArgs &a, const int *walkers, int num, hipStream_t st);
+hipError_t launch_mdet_assemble(const MdetArgs &a, const int *walkers, int num, hipStream_t st);
+template <typename TS, typename TO>
+hipError_t launch_mdet_commit(const MdetArgs &a, const int *walkers, const uint8_t *mask, int num,
+                              hipStream_t st);
+
 // copy whole walker states slot -> slot (DMC branching)
 constexpr int kMaxCopyArrays = 40;
 struct CopyArgs {
@@ -168,6 +219,7 @@ struct ProposeArgs {
   int N, moved, n_up;
   const double *R;
   const double *gradPR[2];
+  const double *cgrad;   // several determinants: [Wc][N][3] grad ln D of the sum (else null)
   const double *Ux;
   double tau;
   uint64_t seed;

@@ -30,10 +30,12 @@ __device__ __forceinline__ double ld(const TS *p) { return (double)*p; }
 template <typename TS, int R, int G>
 __global__ void __launch_bounds__(G * G)
 lu_inverse_kernel(SlaterArgs A, const int *walkers, int num) {
-  const int b = blockIdx.x;
+  const int D = det_count(A), d = blockIdx.x % D;
+  const int b = blockIdx.x / D;
   if (b >= num) return;
-  const int w = walkers ? walkers[b] : b;
-  const int n = A.n;
+  const int w0 = walkers ? walkers[b] : b;
+  const int w = w0 * D + d;  // this determinant's record
+  const int n = A.n, nact = act_count(A);
   const int tid = threadIdx.x;
   // tid = tr + G tc: the G owners of one column are G consecutive lanes
   const int tr = tid % G, tc = tid / G;
@@ -46,7 +48,7 @@ lu_inverse_kernel(SlaterArgs A, const int *walkers, int num) {
   __shared__ int dest[M];
   __shared__ int cur[M];
 
-  const double *dv = A.dval + (long)w * n * n;  // value matrix, (o, e)
+  const double *dv = A.dval + (long)w0 * nact * n;  // value matrix, (o, e)
   double a[R][R];
 #pragma unroll
   for (int i = 0; i < R; i++)
@@ -54,7 +56,7 @@ lu_inverse_kernel(SlaterArgs A, const int *walkers, int num) {
     for (int j = 0; j < R; j++) {
       const int gi = tr + G * i, gj = tc + G * j;  // gi: electron, gj: orbital
       double v = (gi == gj) ? 1.0 : 0.0;
-      if (gi < n && gj < n) v = ld(dv + (long)gj * n + gi);
+      if (gi < n && gj < n) v = ld(dv + (long)occ_row(A, d, gj) * n + gi);
       a[i][j] = v;
     }
   // |det| kept as mantissa * 2^expo to stay in range
@@ -196,10 +198,12 @@ template <typename TS, int NP>
 __global__ void __launch_bounds__(2 * NP)
 gj_rows_kernel(SlaterArgs A, const int *walkers, int num) {
   constexpr int KB = 16, NB = NP / KB, H = NP / 2, NT = 2 * NP, NWV = NT / 64, TR = 32;
-  const int b = blockIdx.x;
+  const int D = det_count(A), d = blockIdx.x % D;
+  const int b = blockIdx.x / D;
   if (b >= num) return;
-  const int w = walkers ? walkers[b] : b;
-  const int n = A.n;
+  const int w0 = walkers ? walkers[b] : b;
+  const int w = w0 * D + d;  // this determinant's record
+  const int n = A.n, nact = act_count(A);
   const int t = threadIdx.x, i = t >> 1, h = t & 1;
   __shared__ double prow[NP], krow[NP];
   __shared__ double wmax[NWV];
@@ -207,12 +211,13 @@ gj_rows_kernel(SlaterArgs A, const int *walkers, int num) {
   __shared__ int ipiv[NP], dest[NP], cur[NP];
   __shared__ double tile[TR][NP + 1];
 
-  const double *dv = A.dval + (long)w * n * n;  // value matrix, (o, e)
+  const double *dv = A.dval + (long)w0 * nact * n;  // value matrix, (o, e)
   double a[H];
 #pragma unroll
   for (int j = 0; j < H; j++) {
     const int col = h * H + j;
-    a[j] = (i < n && col < n) ? ld(dv + (long)col * n + i) : (i == col ? 1.0 : 0.0);
+    a[j] = (i < n && col < n) ? ld(dv + (long)occ_row(A, d, col) * n + i)
+                              : (i == col ? 1.0 : 0.0);
   }
   double mant = 1.0, sgn = 1.0;
   int expo = 0;
@@ -335,14 +340,15 @@ gj_rows_kernel(SlaterArgs A, const int *walkers, int num) {
 template <typename TS>
 hipError_t launch_lu_inverse(const SlaterArgs &a, const int *walkers, int num, hipStream_t st) {
   if (num <= 0 || a.n <= 0) return hipSuccess;
+  const unsigned g = (unsigned)num * (a.ndet > 1 ? a.ndet : 1);
   if (a.n <= 16)
-    hipLaunchKernelGGL((lu_inverse_kernel<TS, 1, 16>), dim3(num), dim3(256), 0, st, a, walkers, num);
+    hipLaunchKernelGGL((lu_inverse_kernel<TS, 1, 16>), dim3(g), dim3(256), 0, st, a, walkers, num);
   else if (a.n <= 32)
-    hipLaunchKernelGGL((lu_inverse_kernel<TS, 2, 16>), dim3(num), dim3(256), 0, st, a, walkers, num);
+    hipLaunchKernelGGL((lu_inverse_kernel<TS, 2, 16>), dim3(g), dim3(256), 0, st, a, walkers, num);
   else if (a.n <= 64)
-    hipLaunchKernelGGL((gj_rows_kernel<TS, 64>), dim3(num), dim3(128), 0, st, a, walkers, num);
+    hipLaunchKernelGGL((gj_rows_kernel<TS, 64>), dim3(g), dim3(128), 0, st, a, walkers, num);
   else if (a.n <= 128)
-    hipLaunchKernelGGL((gj_rows_kernel<TS, 128>), dim3(num), dim3(256), 0, st, a, walkers, num);
+    hipLaunchKernelGGL((gj_rows_kernel<TS, 128>), dim3(g), dim3(256), 0, st, a, walkers, num);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();
@@ -354,23 +360,27 @@ hipError_t launch_lu_inverse(const SlaterArgs &a, const int *walkers, int num, h
 template <typename TS>
 __global__ void __launch_bounds__(256)
 traces_all_kernel(SlaterArgs A, const int *walkers, int num) {
-  const int b = blockIdx.x;
+  const int D = det_count(A), d = blockIdx.x % D;
+  const int b = blockIdx.x / D;
   if (b >= num) return;
-  const int w = walkers ? walkers[b] : b;
-  const int n = A.n;
+  const int w0 = walkers ? walkers[b] : b;
+  const int w = w0 * D + d;  // this determinant's record
+  const int n = A.n, nact = act_count(A);
+  const long cs = (long)nact * n;  // component stride of the derivative matrices
   __shared__ double part[256];
   const double *inv = A.inv + (long)w * n * n;
-  const TS *dm = (const TS *)A.dmat + (long)w * kDer * n * n;
+  const TS *dm = (const TS *)A.dmat + (long)w0 * kDer * cs;
   double lap = 0.0;
   const bool sing = A.singular[w] != 0;
   for (int e = threadIdx.x; e < n; e += blockDim.x) {
     double g0 = 0, g1 = 0, g2 = 0, l = 0;
     for (int o = 0; o < n; o++) {
       const double iv = inv[(long)o * n + e];
-      g0 += ld(dm + (long)o * n + e) * iv;
-      g1 += ld(dm + 1L * n * n + (long)o * n + e) * iv;
-      g2 += ld(dm + 2L * n * n + (long)o * n + e) * iv;
-      l += ld(dm + 3L * n * n + (long)o * n + e) * iv;
+      const long ro = (long)occ_row(A, d, o) * n + e;
+      g0 += ld(dm + ro) * iv;
+      g1 += ld(dm + cs + ro) * iv;
+      g2 += ld(dm + 2 * cs + ro) * iv;
+      l += ld(dm + 3 * cs + ro) * iv;
     }
     if (!sing) {
       double *g = A.gradPR + ((long)w * n + e) * 3;
@@ -390,36 +400,11 @@ traces_all_kernel(SlaterArgs A, const int *walkers, int num) {
 template <typename TS>
 hipError_t launch_traces_all(const SlaterArgs &a, const int *walkers, int num, hipStream_t st) {
   if (num <= 0 || a.n <= 0) return hipSuccess;
-  hipLaunchKernelGGL((traces_all_kernel<TS>), dim3(num), dim3(256), 0, st, a, walkers, num);
+  const unsigned g = (unsigned)num * (a.ndet > 1 ? a.ndet : 1);
+  hipLaunchKernelGGL((traces_all_kernel<TS>), dim3(g), dim3(256), 0, st, a, walkers, num);
   return hipGetLastError();
 }
 
-// ---------------------------------------------------------------------------
-// Drift modification of one electron, umrigar93_unequalelectrons
-// (QMCSCFJastrow.cpp:368-443).
-__device__ inline double drift_factor_unequal(const double g[3], const double x[3], int nn,
-                                              const double *nuc_xyz, const double *nuc_z,
-                                              double dt) {
-  int cn = 0;
-  double best = 1e300;
-  for (int A = 0; A < nn; A++) {
-    const double dx = x[0] - nuc_xyz[3 * A], dy = x[1] - nuc_xyz[3 * A + 1],
-                 dz = x[2] - nuc_xyz[3 * A + 2];
-    const double r = sqrt(dx * dx + dy * dy + dz * dz);
-    if (r < best) { best = r; cn = A; }
-  }
-  double nv[3], dsq = 0.0;
-  for (int k = 0; k < 3; k++) { nv[k] = x[k] - nuc_xyz[3 * cn + k]; dsq += nv[k] * nv[k]; }
-  const double inr = 1.0 / sqrt(dsq);
-  const double v2 = g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
-  if (fabs(v2) < 1e-10) return 1.0;
-  const double iv = 1.0 / sqrt(v2);
-  const double dotp = (nv[0] * g[0] + nv[1] * g[1] + nv[2] * g[2]) * inr * iv;
-  const double Z = nuc_z[cn];
-  const double t = Z * Z * dsq;
-  const double a = 0.5 * (1.0 + dotp) + t / (10.0 * (4.0 + t));
-  return (-1.0 + sqrt(1.0 + 2.0 * a * v2 * dt)) / (a * v2 * dt);
-}
 
 // ---------------------------------------------------------------------------
 // Single-electron move of spin s, one walker per workgroup, one streaming pass.

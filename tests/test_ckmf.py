@@ -34,3 +34,23 @@ def test_reader_shapes():
     assert packed["n_up"] == 76 and packed["n_dn"] == 76
     assert packed["n_basis"] == 516 and packed["n_nuc"] == 28
     assert packed["mo_up"].shape == (76, 516)
+
+
+def test_reader_multideterminant_tables():
+    """The Be MCSCF example: 10 determinants over 5 orbitals; each
+    determinant's columns are its occupied orbitals in orbital order among the
+    active ones (QMCWavefunction::makeCoefficients)."""
+    import gzip, os, tempfile
+    from qmcbeaver_amd import ckmf
+    from tests.inputs_util import GOLDEN
+    with tempfile.TemporaryDirectory() as tmp:
+        raw = os.path.join(tmp, "be.ckmf")
+        with gzip.open(os.path.join(GOLDEN, "be_mcscf.ckmf.gz"), "rb") as g, open(raw, "wb") as f:
+            f.write(g.read())
+        pk, flags, info = ckmf.read_ckmf(raw)
+    assert pk["n_det"] == 10 and pk["n_up"] == 2 and pk["n_dn"] == 2
+    assert pk["n_act_up"] == 5 and pk["mo_up"].shape == (5, 9)
+    assert pk["occ_up"].tolist()[:4] == [[0, 1], [0, 3], [0, 2], [0, 4]]
+    assert pk["occ_dn"].tolist()[4] == [0, 4]
+    assert abs(pk["ci_coef"][0] - 0.9497089) < 1e-12
+    assert abs(pk["ci_coef"].sum() - (0.9497089 - 3 * 0.1807882 + 0.0000002)) < 1e-6

@@ -1,0 +1,129 @@
+"""Several determinants on the device against the reference build
+(tests/golden/make_golden_multidet.py, make_golden_vmc.py): the reference's
+MCSCF examples Be (10 determinants) and CH2 (2 determinants), with and
+without cusp replacement.
+
+* all-electron evaluates: <= 1e-10 relative (ln|Psi|, E_L, KE, V, grad ln Psi,
+  sign exact);
+* single-electron move sequences (one Sherman-Morrison update per
+  determinant, the determinant sum recombined): <= 1e-9;
+* device VMC: after hundreds of moves the resident per-determinant state
+  equals a fresh all-electron evaluate, and the Be VMC energy agrees with the
+  reference's within 4 combined DDDA error bars.
+"""
+import gzip
+import json
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+from qmcbeaver_amd import Context, ckmf
+from qmcbeaver_amd.stats import Property, series_energy
+from tests.inputs_util import GOLDEN, configs
+
+pytestmark = pytest.mark.gpu
+CASES = [("be_mcscf", "0"), ("be_mcscf", "1"), ("ch2_mcscf", "0"), ("ch2_mcscf", "1")]
+
+
+def rel(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    return float(np.max(np.abs(a - b) / np.maximum(1.0, np.abs(b))))
+
+
+def load(name):
+    with tempfile.TemporaryDirectory() as tmp:
+        raw = os.path.join(tmp, name + ".ckmf")
+        with gzip.open(os.path.join(GOLDEN, name + ".ckmf.gz"), "rb") as g, open(raw, "wb") as f:
+            f.write(g.read())
+        packed, _, _ = ckmf.read_ckmf(raw)
+    return packed
+
+
+@pytest.mark.parametrize("name,cusp", CASES)
+def test_multidet_all_electron_matches_reference(name, cusp):
+    g = np.load(os.path.join(GOLDEN, f"multidet_{name}.npz"))
+    pk = load(name)
+    R = g["R"]
+    t = f"c{cusp}_ev_"
+    print("case multidet all-electron", name, "cusp", cusp, "determinants", pk["n_det"], flush=True)
+    with Context(pk, precision="fp64", max_walkers=R.shape[0], replace_cusps=cusp == "1") as ctx:
+        res = ctx.evaluate(np.arange(R.shape[0]), R)
+    assert not res["singular"].any()
+    for k in ("log_psi", "e_local", "kinetic", "v_ne", "v_ee"):
+        assert rel(res[k], g[t + k]) < 1e-10, (k, rel(res[k], g[t + k]))
+    assert rel(res["grad_lnpsi"], g[t + "grad_lnpsi"]) < 1e-10
+    np.testing.assert_array_equal(res["sign_psi"], g[t + "sign_psi"].astype(np.int8))
+
+
+@pytest.mark.parametrize("name,cusp", CASES)
+def test_multidet_single_electron_sequence_matches_reference(name, cusp):
+    g = np.load(os.path.join(GOLDEN, f"multidet_{name}.npz"))
+    t = f"c{cusp}_sq_"
+    if t + "e_local" not in g:
+        pytest.skip("no reference sequence")
+    pk = load(name)
+    mv = g["moves"]
+    nw, nm = mv.shape[:2]
+    R = g["R"][:nw].copy()
+    nrec = nm + 1
+    print("case multidet single-electron", name, "cusp", cusp, flush=True)
+    with Context(pk, precision="fp64", max_walkers=nw, replace_cusps=cusp == "1") as ctx:
+        ids = np.arange(nw)
+        res = ctx.evaluate(ids, R)
+        assert rel(res["e_local"], g[t + "e_local"][np.arange(nw) * nrec]) < 1e-10
+        for m in range(nm):
+            e = int(mv[0, m, 0])
+            R[:, e, :] = mv[:, m, 1:]
+            res = ctx.evaluate(ids, R, whichE=e)
+            idx = np.arange(nw) * nrec + m + 1
+            assert rel(res["log_psi"], g[t + "log_psi"][idx]) < 1e-9, m
+            assert rel(res["e_local"], g[t + "e_local"][idx]) < 1e-9, m
+            assert rel(res["grad_lnpsi"], g[t + "grad_lnpsi"][idx]) < 1e-9, m
+            ctx.accept(ids, np.ones(nw, dtype=np.uint8))
+
+
+def test_multidet_rejected_proposals_and_vmc_state():
+    pk = load("be_mcscf")
+    W = 64
+    R0 = configs(pk, W, 61)
+    print("case multidet VMC resident state (Be)", flush=True)
+    with Context(pk, precision="fp64", max_walkers=W, seed=7, replace_cusps=True,
+                 refresh_interval=0) as ctx, \
+            Context(pk, precision="fp64", max_walkers=W, replace_cusps=True) as fresh:
+        ids = np.arange(W)
+        ctx.evaluate(ids, R0)
+        st = ctx.vmc_steps(W, 0, 400, 0.02)
+        assert 0.5 < st.sum_accept / (W * 400) < 1.0
+        R = ctx.positions(W)
+        ref = fresh.evaluate(ids, R)
+        for e in (0, 1, 2, 3):
+            res = ctx.evaluate(ids, R, whichE=e)  # zero move against the resident state
+            assert np.allclose(res["det_ratio"], 1.0, rtol=1e-9, atol=0)
+            assert rel(res["e_local"], ref["e_local"]) < 1e-8
+            assert rel(res["log_psi"], ref["log_psi"]) < 1e-8
+            ctx.accept(ids, np.zeros(W, dtype=np.uint8))
+
+
+def test_multidet_vmc_energy_matches_reference():
+    with open(os.path.join(GOLDEN, "vmc_be_mcscf.json")) as f:
+        g = json.load(f)
+    pk = load("be_mcscf")
+    W, sweeps, tau = g["walkers"], g["sweeps"], g["tau"]
+    R0 = configs(pk, W, g["config_seed"])
+    N = R0.shape[1]
+    print("case multidet VMC energy (Be MCSCF)", W, "walkers", sweeps, "sweeps", flush=True)
+    with Context(pk, precision="fp64", max_walkers=W, seed=31, replace_cusps=True) as ctx:
+        ctx.evaluate(np.arange(W), R0)
+        p = Property()
+        for blk in range(3):
+            n = sweeps // 3
+            ctx.vmc_steps(W, blk * n * N, n * N, tau)
+            series_energy(ctx.sample_series(), p)
+    s = p.summary()
+    ref = g["reference"]
+    print("device", s.average, "+-", s.std_dev, "reference", ref["ddda_energy"], "+-",
+          ref["ddda_std"], flush=True)
+    assert s.decorr_depth >= 0
+    assert abs(s.average - ref["ddda_energy"]) < 4 * np.hypot(s.std_dev, ref["ddda_std"])
