@@ -186,6 +186,43 @@ def cpu_baseline(s, tau, seed, target_s=15.0):
             "sample": f"{nw} walkers x {moves} single-electron VMC moves (C oracle), {t:.1f} s"}
 
 
+def serving_run(precision, n_jobs, rate, seed, workers, capacity_gib):
+    """Serving loop: Poisson arrivals of VMC/DMC jobs (8-256 electrons,
+    1e3-1e5 walker-moves each, blocks of 100 steps), continuous batching
+    with admission by projected device footprint."""
+    from qmcbeaver_amd import serving
+    jobs = serving.poisson_trace(n_jobs, rate, seed)
+    srv = serving.Server(capacity_bytes=int(capacity_gib * (1 << 30)), workers=workers,
+                         runner=serving.JobRunner(precision=precision))
+    print(f"serving {precision}: {n_jobs} jobs at {rate}/s", file=sys.stderr, flush=True)
+    out = srv.run(jobs)
+    print(f"serving {precision}: {out['completed']} completed in {out['wall_s']:.1f} s",
+          file=sys.stderr, flush=True)
+    out["failed"] = out["failed"][:5]
+    out["precision"] = precision
+    out["arrival_rate_per_s"] = rate
+    out["mean_jobs_in_flight"] = float(sum(j.finished - j.admitted for j in srv.done
+                                           if j.admitted is not None) / max(out["wall_s"], 1e-9))
+    return out
+
+
+def batched_run(precision, n_mol, steps, warmup, seed, workers, tau):
+    """Batched uneven DMC: n_mol molecules (8-200 electrons, B ~ 5 N,
+    256-4096 walkers each), DMC with tau, all molecules advanced together."""
+    from qmcbeaver_amd import serving
+    comp = serving.batched_composition(n_mol, seed)
+    with serving.BatchedDMC(comp, precision=precision, tau=tau, workers=workers) as b:
+        fp = b.footprint()
+        print(f"batched {precision}: {n_mol} molecules, {sum(c[1] for c in comp)} walkers, "
+              f"projected {fp / 2**30:.1f} GiB", file=sys.stderr, flush=True)
+        out = b.run(steps, warmup)
+        print(f"batched {precision}: setup {out['setup_s']:.1f} s, timed {out['wall_s']:.2f} s",
+              file=sys.stderr, flush=True)
+    out["projected_device_bytes"] = fp
+    out["precision"] = precision
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -201,6 +238,17 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--accuracy", type=int, default=10000,
                     help="configurations for the mixed-vs-FP64 error report (0 = skip)")
+    ap.add_argument("--workload", default="mol256", choices=["mol256", "batched", "serving"],
+                    help="mol256: one 256-electron molecule (default); batched: uneven DMC "
+                         "over many molecules; serving: Poisson arrival stream of jobs")
+    ap.add_argument("--serving-jobs", type=int, default=200,
+                    help="jobs in the serving trace (default 200 = 10 s of arrivals at 20/s; "
+                         "0 = no serving run beside the mol256 workload)")
+    ap.add_argument("--rate", type=float, default=20.0, help="serving arrivals per second")
+    ap.add_argument("--serving-workers", type=int, default=8)
+    ap.add_argument("--capacity-gib", type=float, default=160.0,
+                    help="device memory the serving loop may hand to jobs")
+    ap.add_argument("--molecules", type=int, default=64)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy three times (the "
                          "reference kernel for scaling the FETCH_SIZE / WRITE_SIZE counters)")
@@ -219,6 +267,45 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU: the walker-evaluation path "
                                    "is measured at N=1 only (see DESIGN.md for how it shards)"}))
         sys.exit(2)
+
+    if args.workload == "serving":
+        mixed = serving_run(args.precision, args.serving_jobs, args.rate, 20, args.serving_workers,
+                            args.capacity_gib)
+        fp64 = None if args.no_fp64 else serving_run("fp64", args.serving_jobs, args.rate, 20,
+                                                     args.serving_workers, args.capacity_gib)
+        print(json.dumps({
+            "metric": "aggregate walker-moves/s + p50/p99 block latency (serving loop)",
+            "value": mixed["walker_moves_per_s"], "unit": "walker-moves/s", "n_gpus": 1,
+            "steps": args.serving_jobs, "warmup": 0,
+            "ms_per_step": 1e3 * mixed["wall_s"] / max(args.serving_jobs, 1),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32+f64" if args.precision != "fp64" else "f64", "data": "synthetic",
+            "config": {"workload": f"serving loop: {args.serving_jobs} VMC/DMC jobs, Poisson "
+                                   f"arrivals {args.rate}/s, 8-256 e-, 1e3-1e5 walker-moves per "
+                                   "job in blocks of 100 steps, admission by projected device "
+                                   f"footprint ({args.capacity_gib} GiB), "
+                                   f"{args.serving_workers} job slots"},
+            "block_latency_ms": mixed["block_latency_ms"], "serving": mixed, "fp64": fp64}))
+        return
+    if args.workload == "batched":
+        mixed = batched_run(args.precision, args.molecules, args.steps, args.warmup, 0x51EED03,
+                            16, 0.01)
+        fp64 = None if args.no_fp64 else batched_run("fp64", args.molecules, args.steps,
+                                                     args.warmup, 0x51EED03, 16, 0.01)
+        print(json.dumps({
+            "metric": "aggregate walker-moves/s (batched uneven DMC)",
+            "value": mixed["walker_moves_per_s"], "unit": "walker-moves/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": 1e3 * mixed["wall_s"] / max(args.steps, 1),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32+f64" if args.precision != "fp64" else "f64", "data": "synthetic",
+            "config": {"workload": f"batched uneven DMC: {args.molecules} molecules, 8-200 e-, "
+                                   "B ~ 5 N, 256-4096 walkers each, tau=0.01, one step = one "
+                                   "single-electron DMC step of every molecule"},
+            "batched": mixed, "fp64": fp64,
+            "mixed_over_fp64": None if fp64 is None else
+            mixed["walker_moves_per_s"] / fp64["walker_moves_per_s"]}))
+        return
 
     from qmcbeaver_amd import synth
     s = synth.make_system(args.config)
@@ -302,6 +389,23 @@ def main():
         if store is not None:
             out["fp32store"]["error"] = accuracy_report(s, args.accuracy, tau=args.tau,
                                                         precision="fp32store")
+    if args.serving_jobs > 0:
+        srv = serving_run(args.precision, args.serving_jobs, args.rate, 20, args.serving_workers,
+                          args.capacity_gib)
+        srv64 = None if args.no_fp64 else serving_run(
+            "fp64", args.serving_jobs, args.rate, 20, args.serving_workers, args.capacity_gib)
+        out["serving"] = {
+            "workload": f"{args.serving_jobs} VMC/DMC jobs, Poisson arrivals {args.rate}/s, "
+                        "8-256 e-, 1e3-1e5 walker-moves per job in blocks of 100 steps, "
+                        f"{args.serving_workers} job slots, admission by projected footprint",
+            "block_latency_ms": srv["block_latency_ms"],
+            "walker_moves_per_s": srv["walker_moves_per_s"], "wall_s": srv["wall_s"],
+            "completed": srv["completed"], "failed": srv["failed"], "rejected": srv["rejected"],
+            "peak_active": srv["peak_active"], "peak_bytes": srv["peak_bytes"],
+            "fp64": None if srv64 is None else {
+                "block_latency_ms": srv64["block_latency_ms"],
+                "walker_moves_per_s": srv64["walker_moves_per_s"], "wall_s": srv64["wall_s"],
+                "completed": srv64["completed"], "failed": srv64["failed"]}}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(s, args.tau, 99)
     print(json.dumps(out))

@@ -113,6 +113,9 @@ typedef struct qmcb_options {
   int32_t device;           /* HIP device ordinal                              */
   int32_t replace_cusps;    /* 1: electron-nucleus cusp replacement of the
                                orbitals (replace_electron_nucleus_cusps 1)     */
+  int64_t basis_scratch_bytes; /* cap on the basis-value scratch; 0 = 2 GiB.
+                               All-electron evaluates run in chunks that fit;
+                               at least one point per walker is kept          */
 } qmcb_options;
 
 /* Caller-owned outputs, one row per evaluated walker.  Any pointer may be
@@ -192,7 +195,12 @@ qmcb_status qmcb_get_positions(qmcb_ctx *ctx, int32_t num, double *R);
  * QMCWalker.cpp:179-282, 343-398, 756-780, 1578-1685): one step moves
  * electron (step mod N) of every walker in slots [0, num), evaluates the
  * full contract at the proposal, and accepts or rejects on the device with a
- * counter-based RNG keyed by (seed, slot, step). */
+ * counter-based RNG keyed by (seed, slot, step).  The drift is gradPsiRatio
+ * (qmcb_options.drift_mod QMCB_DRIFT_NONE) or, with one determinant, its
+ * umrigar93_unequalelectrons modification (QMCB_DRIFT_UMRIGAR93_UNEQUAL, the
+ * reference's default QF_modification_type; QMCSCFJastrow.cpp:368-443) with
+ * the step's tau as its time step, in the proposal and the Green's function
+ * ratio alike.  Same for qmcb_dmc_steps. */
 typedef struct qmcb_vmc_stats {
   double sum_e_local;       /* sum over walkers and steps of E_L (current cfg) */
   double sum_e_local_sq;
@@ -222,6 +230,11 @@ typedef struct qmcb_dmc_params {
   double fusion_threshold;  /* fuse walkers below this weight (0.45)          */
   int32_t target_walkers;   /* population the E_T feedback aims at            */
   int32_t reweight_after;   /* weights stay 1 before this step (100)          */
+  int32_t energy_modification; /* 0: energy_modification_type none,
+                               1: umrigar93 -- S = (E_T - E_est) + (E_est - E_L)
+                               |modified drift| / |drift| (QMCWalker.cpp:1121-
+                               1132); needs qmcb_options.drift_mod
+                               QMCB_DRIFT_UMRIGAR93_UNEQUAL to differ from 0   */
 } qmcb_dmc_params;
 
 typedef struct qmcb_dmc_stats {
@@ -245,6 +258,20 @@ qmcb_status qmcb_dmc_population(qmcb_ctx *ctx, int32_t cap, int32_t *slots,
                                 double *weights, int32_t *count);
 
 qmcb_status qmcb_sync(qmcb_ctx *ctx);
+
+/* ---- device memory ------------------------------------------------------
+ * qmcb_device_bytes: device memory the context holds (requested sizes).
+ * qmcb_plan_device_bytes: host-only projection of the most a context of
+ * this system and these options allocates: qmcb_create plus the staging
+ * arrays of the first single-electron qmcb_evaluate (or, with several
+ * determinants, the first move) (cusp tables at
+ * their upper bound; the per-sweep series buffer, 32 bytes per recorded
+ * sweep, excluded), for admission control of many contexts on one device.
+ * No reference counterpart: the reference sizes host memory per walker
+ * (QMCWalker.cpp) and runs one molecule per process. */
+qmcb_status qmcb_device_bytes(const qmcb_ctx *ctx, int64_t *bytes);
+qmcb_status qmcb_plan_device_bytes(const qmcb_system *sys, const qmcb_options *opt,
+                                   int64_t *bytes);
 
 /* ---- electron-nucleus cusp replacement ---------------------------------
  * With qmcb_options.replace_cusps = 1 the s-type part of every orbital near

@@ -43,7 +43,8 @@ class Options(C.Structure):
     _fields_ = [("precision", C.c_int32), ("max_walkers", C.c_int32),
                 ("drift_mod", C.c_int32), ("refresh_interval", C.c_int32),
                 ("dt", C.c_double), ("umrigar_a", C.c_double),
-                ("seed", C.c_uint64), ("device", C.c_int32), ("replace_cusps", C.c_int32)]
+                ("seed", C.c_uint64), ("device", C.c_int32), ("replace_cusps", C.c_int32),
+                ("basis_scratch_bytes", C.c_int64)]
 
 
 class Results(C.Structure):
@@ -68,7 +69,8 @@ class VmcStats(C.Structure):
 class DmcParams(C.Structure):
     _fields_ = [("tau", C.c_double), ("e_trial", C.c_double), ("pop_control", C.c_double),
                 ("branch_threshold", C.c_double), ("fusion_threshold", C.c_double),
-                ("target_walkers", C.c_int32), ("reweight_after", C.c_int32)]
+                ("target_walkers", C.c_int32), ("reweight_after", C.c_int32),
+                ("energy_modification", C.c_int32)]
 
 
 class DmcStats(C.Structure):

@@ -26,7 +26,8 @@ EXPORTS = ["qmcb_abi_version", "qmcb_create", "qmcb_destroy", "qmcb_last_error",
            "qmcb_dmc_population", "qmcb_sample_series", "qmcb_property_create",
            "qmcb_property_destroy", "qmcb_property_zero", "qmcb_property_add",
            "qmcb_property_add_series", "qmcb_property_merge", "qmcb_property_reweight",
-           "qmcb_property_summary_get", "qmcb_property_level", "qmcb_cusp_fit"]
+           "qmcb_property_summary_get", "qmcb_property_level", "qmcb_cusp_fit",
+           "qmcb_device_bytes", "qmcb_plan_device_bytes"]
 
 _lib = None
 
@@ -81,6 +82,11 @@ def lib():
     L.qmcb_dmc_population.argtypes = [vp, C.c_int32, ip, dp, C.POINTER(C.c_int32)]
     L.qmcb_cusp_fit.restype = C.c_int
     L.qmcb_cusp_fit.argtypes = [C.POINTER(SystemStruct), C.c_int32, C.POINTER(CuspOrbital)]
+    L.qmcb_device_bytes.restype = C.c_int
+    L.qmcb_device_bytes.argtypes = [vp, C.POINTER(C.c_int64)]
+    L.qmcb_plan_device_bytes.restype = C.c_int
+    L.qmcb_plan_device_bytes.argtypes = [C.POINTER(SystemStruct), C.POINTER(Options),
+                                         C.POINTER(C.c_int64)]
     L.qmcb_sample_series.restype = C.c_int
     L.qmcb_sample_series.argtypes = [vp, C.c_int32, dp, C.POINTER(C.c_int32)]
     L.qmcb_property_create.restype = vp
@@ -118,6 +124,37 @@ def cusp_fit(packed: Dict, spin: int = 0) -> np.ndarray:
     return out.reshape(nn, norb)
 
 
+def _options(precision, max_walkers, drift_mod=QMCB_DRIFT_NONE, dt=0.1, umrigar_a=0.5,
+             refresh_interval=0, seed=12345, device=0, replace_cusps=False,
+             basis_scratch_bytes=0) -> Options:
+    o = Options()
+    o.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+    o.max_walkers = max_walkers
+    o.drift_mod = drift_mod
+    o.refresh_interval = refresh_interval
+    o.dt = dt
+    o.umrigar_a = umrigar_a
+    o.seed = seed
+    o.device = device
+    o.replace_cusps = 1 if replace_cusps else 0
+    o.basis_scratch_bytes = int(basis_scratch_bytes)
+    return o
+
+
+def plan_device_bytes(packed: Dict, *, precision: str = "fp64", max_walkers: int = 1,
+                      replace_cusps: bool = False, basis_scratch_bytes: int = 0) -> int:
+    """Device bytes a Context of this system and these options will hold
+    after its first evaluate (host-only projection, no device needed)."""
+    buf = SystemBuffers(packed)
+    o = _options(precision, max_walkers, replace_cusps=replace_cusps,
+                 basis_scratch_bytes=basis_scratch_bytes)
+    b = C.c_int64()
+    st = lib().qmcb_plan_device_bytes(C.byref(buf.struct), C.byref(o), C.byref(b))
+    if st != 0:
+        raise QmcbError(f"qmcb_plan_device_bytes: {STATUS.get(st, st)}")
+    return int(b.value)
+
+
 def _ip(a):
     return a.ctypes.data_as(C.POINTER(C.c_int32))
 
@@ -137,21 +174,13 @@ class Context:
     def __init__(self, packed: Dict, *, precision: str = "fp64", max_walkers: int = 1,
                  drift_mod: int = QMCB_DRIFT_NONE, dt: float = 0.1, umrigar_a: float = 0.5,
                  refresh_interval: int = 0, seed: int = 12345, device: int = 0,
-                 replace_cusps: bool = False):
+                 replace_cusps: bool = False, basis_scratch_bytes: int = 0):
         self._buf = SystemBuffers(packed)
         self.N = self._buf.N
         self.n_up = int(packed["n_up"])
         self.max_walkers = max_walkers
-        o = Options()
-        o.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
-        o.max_walkers = max_walkers
-        o.drift_mod = drift_mod
-        o.refresh_interval = refresh_interval
-        o.dt = dt
-        o.umrigar_a = umrigar_a
-        o.seed = seed
-        o.device = device
-        o.replace_cusps = 1 if replace_cusps else 0
+        o = _options(precision, max_walkers, drift_mod, dt, umrigar_a, refresh_interval, seed,
+                     device, replace_cusps, basis_scratch_bytes)
         self._opt = o
         h = C.c_void_p()
         st = lib().qmcb_create(C.byref(self._buf.struct), C.byref(o), C.byref(h))
@@ -216,9 +245,12 @@ class Context:
 
     def dmc_steps(self, first_step: int, n_steps: int, tau: float, e_trial: float, *,
                   target: int = 0, pop_control: float = 1.0, branch_threshold: float = 2.0,
-                  fusion_threshold: float = 0.45, reweight_after: int = 100) -> DmcStats:
+                  fusion_threshold: float = 0.45, reweight_after: int = 100,
+                  energy_modification: bool = False) -> DmcStats:
+        """energy_modification: the reference's umrigar93 energy modification
+        of the weights (meaningful with drift_mod=QMCB_DRIFT_UMRIGAR93_UNEQUAL)."""
         p = DmcParams(tau, e_trial, pop_control, branch_threshold, fusion_threshold, target,
-                      reweight_after)
+                      reweight_after, 1 if energy_modification else 0)
         st = DmcStats()
         self._check(lib().qmcb_dmc_steps(self._h, first_step, n_steps, C.byref(p),
                                          C.byref(st)), "qmcb_dmc_steps")
@@ -247,6 +279,12 @@ class Context:
 
     def sync(self):
         self._check(lib().qmcb_sync(self._h), "qmcb_sync")
+
+    def device_bytes(self) -> int:
+        """Device memory this context holds."""
+        b = C.c_int64()
+        self._check(lib().qmcb_device_bytes(self._h, C.byref(b)), "qmcb_device_bytes")
+        return int(b.value)
 
     def enable_timing(self, on: bool = True):
         lib().qmcb_enable_kernel_timing(self._h, 1 if on else 0)

@@ -307,10 +307,16 @@ __global__ void __launch_bounds__(256) propose_kernel(ProposeArgs A, const int *
   const double r1 = sqrt(-2.0 * log(u[0])), r2 = sqrt(-2.0 * log(u[2]));
   const double g[3] = {r1 * cos(two_pi * u[1]), r1 * sin(two_pi * u[1]), r2 * cos(two_pi * u[3])};
   const double st = sqrt(A.tau);
+  double v[3] = {gp[0] + ux[0], gp[1] + ux[1], gp[2] + ux[2]};
+  if (A.drift_mod == QMCB_DRIFT_UMRIGAR93_UNEQUAL) {
+    const double xm[3] = {x[0], x[1], x[2]};
+    const double f = drift_factor_unequal(v, xm, A.n_nuc, A.nuc_xyz, A.nuc_z, A.tau);
+    for (int k = 0; k < 3; k++) v[k] *= f;
+  }
   for (int k = 0; k < 3; k++) {
     const double chi = st * g[k];
     A.gauss[3L * b + k] = chi;
-    A.propR[3L * b + k] = x[k] + A.tau * (gp[k] + ux[k]) + chi;
+    A.propR[3L * b + k] = x[k] + A.tau * v[k] + chi;
   }
   A.unif[b] = u[4];
 }

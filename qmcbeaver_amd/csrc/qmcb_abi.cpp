@@ -47,6 +47,7 @@ struct qmcb_ctx {
   int Wc = 0;
   DevSystem dsys{};
   std::vector<void *> allocs;
+  size_t dev_bytes = 0;  // device memory requested by this context
   size_t ts = 8, tx = 8, to = 8;  // bytes: derivative storage, GEMM operand, GEMM output
 
   // resident walker state
@@ -105,6 +106,7 @@ struct qmcb_ctx {
   bool dmc_ready = false;
   bool have_e_trial = false;
   double e_trial = 0.0;
+  double e_est = 0.0;  // weighted mean local energy the trial energy is fed back from
   // staged single-electron pass
   int staged_moved = -1;
   int staged_num = 0;
@@ -139,6 +141,7 @@ hipError_t dalloc(qmcb_ctx *ctx, T **p, size_t count) {
   hipError_t e = hipMalloc(&q, bytes);
   if (e != hipSuccess) return e;
   ctx->allocs.push_back(q);
+  ctx->dev_bytes += bytes;
   e = hipMemsetAsync(q, 0, bytes, ctx->stream);
   *p = (T *)q;
   return e;
@@ -149,6 +152,7 @@ hipError_t dalloc_bytes(qmcb_ctx *ctx, void **p, size_t bytes) {
   hipError_t e = hipMalloc(p, bytes);
   if (e != hipSuccess) return e;
   ctx->allocs.push_back(*p);
+  ctx->dev_bytes += bytes;
   return hipMemsetAsync(*p, 0, bytes, ctx->stream);
 }
 
@@ -735,7 +739,8 @@ qmcb_status create_impl(const qmcb_system *s, const qmcb_options *o, qmcb_ctx *c
   QCHK(dalloc(ctx, &ctx->d_mask, W));
   // basis scratch: at least one point per walker; all-electron chunks reuse it
   const size_t per_point = (size_t)kComp * ctx->kpad * ctx->tx;
-  const size_t budget = (size_t)1 << 31;  // 2 GiB of basis values per chunk
+  const size_t budget = o->basis_scratch_bytes > 0 ? (size_t)o->basis_scratch_bytes
+                                                   : (size_t)1 << 31;  // basis values per chunk
   long cap = std::max<long>(W, std::min<long>(W * std::max(ctx->n[0], 1), (long)(budget / per_point)));
   cap = std::max<long>(cap, std::max(ctx->n[0], 1));
   ctx->ldp = ((cap + 63) / 64) * 64;
@@ -819,6 +824,118 @@ qmcb_status evaluate_core(qmcb_ctx *ctx, int num, const int *d_ids, const double
   if (ctx->multi) return DISPATCH(ctx, mdet_stage(ctx, d_ids, num, whichE, 0, dout, 0.0, nullptr));
   MoveArgs a = DISPATCH(ctx, move_args(ctx, whichE, 0, dout));
   return DISPATCH(ctx, sm(ctx, a, d_ids, num));
+}
+
+// The most device memory a context of this system and these options
+// requests: the arrays create_impl allocates plus the staging arrays
+// (ensure_staging: first single-electron evaluate, or first move with
+// several determinants), with the cusp tables at their upper bound (every primitive an s
+// Gaussian).  The per-sweep series buffer (32 bytes per recorded sweep) is
+// not included.
+size_t plan_device_bytes(const qmcb_system *s, const qmcb_options *o) {
+  size_t total = 0;
+  auto add = [&](size_t count, size_t elem) { total += std::max<size_t>(count * elem, 16); };
+  const bool f64 = o->precision == QMCB_FP64;
+  const bool store32 = o->precision == QMCB_MIXED_F32STORE;
+  const size_t ts = f64 ? 8 : 4, tx = (f64 || store32) ? 8 : 4, to = tx;
+  const long W = o->max_walkers, N = s->n_up + s->n_dn, nn = s->n_nuc, B = s->n_basis;
+  const long kpad = ((B + 31) / 32) * 32;
+  const long n[2] = {s->n_up, s->n_dn};
+  // system tables
+  const Shells sh = make_shells(s);
+  const long nshell = (long)sh.atom.size();
+  add(3 * nn, 8);
+  add(nn, 8);
+  for (int i = 0; i < 5; i++) add(nshell, 4);
+  add(3 * B, 4);
+  add(std::max(1L, std::min(32L, nshell / 4)) + 1, 4);
+  add(s->n_prim, 8);
+  add(s->n_prim, 8);
+  add(nn, sizeof(qmcb_pade));
+  add(nn, sizeof(qmcb_pade));
+  const bool multi = s->n_det > 1 || s->occ_up || s->occ_dn;
+  const long ndet = std::max(1, (int)s->n_det);
+  long nact[2];
+  for (int sp = 0; sp < 2; sp++) {
+    const int na = sp == 0 ? s->n_act_up : s->n_act_dn;
+    nact[sp] = (multi && na > 0) ? na : n[sp];
+    if (multi && n[sp] > 0) add(ndet * n[sp], 4);
+  }
+  if (multi) add(ndet, 8);
+  long npad[2];
+  for (int sp = 0; sp < 2; sp++) {
+    npad[sp] = pad_orb((int)std::max(nact[sp], 1L));
+    add(kpad * npad[sp], tx);
+  }
+  if (o->replace_cusps)
+    for (int sp = 0; sp < 2; sp++) {
+      if (n[sp] == 0) continue;
+      add(nn, 8);
+      add(nn + 1, 4);
+      add(s->n_prim, 8);
+      add(nact[sp] * s->n_prim, 8);
+      add(nn * nact[sp] * 8, 8);
+    }
+  // resident walker state
+  const long WD = W * ndet;
+  add(W * N * 3, 8);
+  for (int sp = 0; sp < 2; sp++) {
+    add(WD * n[sp] * n[sp], 8);
+    add(W * nact[sp] * n[sp], 8);
+    add((size_t)W * kDer * nact[sp] * n[sp], ts);
+    add(WD, 8);
+    add(WD, 8);
+    add(WD * std::max(n[sp], 1L) * 3, 8);
+    add(WD, 8);
+    add(WD, 4);
+    add(WD, 4);
+    add(W * kDelay * std::max(n[sp], 1L), 8);
+    add(W * kDelay * std::max(n[sp], 1L), 8);
+  }
+  add(W, 8);
+  add(W * N * 3, 8);
+  if (multi) add(W * N * 3, 8);
+  for (int i = 0; i < 4; i++) add(W, 8);   // Uxx, Vee, Ven, cur_el
+  add(W * N * 3, 8);                         // Rin
+  add(W * 3, 8);
+  add(W * 3, 8);
+  add(W, 8);
+  add(W, 8);
+  add(W * N * 3, 8);
+  for (int i = 0; i < 3; i++) add(W, 8);   // pUxx, pVee, pVen
+  const long ldv = ((W + 63) / 64) * 64;
+  add((size_t)kComp * std::max(npad[0], npad[1]) * ldv, to);
+  for (int i = 0; i < 4; i++) add(W, 8);   // energy accumulators
+  add(3 * W, 8);
+  add(W, 8);
+  add(W, 8);
+  add(W, 4);
+  add(2 * W, 4);
+  add(W, 4);
+  add(W, 1);
+  const size_t per_point = (size_t)kComp * kpad * tx;
+  const size_t budget = o->basis_scratch_bytes > 0 ? (size_t)o->basis_scratch_bytes
+                                                   : (size_t)1 << 31;
+  long cap = std::max<long>(W, std::min<long>(W * std::max(n[0], 1L), (long)(budget / per_point)));
+  cap = std::max<long>(cap, std::max(n[0], 1L));
+  add(per_point * (size_t)(((cap + 63) / 64) * 64), 1);
+  add(W, 8);                                 // log_psi
+  add(W, 1);                                 // sign_psi
+  add(W * N * 3, 8);
+  add(W * N * 3, 8);
+  for (int i = 0; i < 8; i++) add(W, 8);   // lap_ratio .. jastrow_u
+  add(W, 1);                                 // singular
+  // staging (first single-electron evaluate or move)
+  const long nm = std::max(n[0], n[1]);
+  add(WD * nm * nm, 8);
+  add(WD * nm * 3, 8);
+  add(WD, 8);
+  add(WD, 8);
+  add(WD, 8);
+  add(WD, 4);
+  add(W, 4);
+  if (multi) add(W * N * 3, 8);
+  return total;
 }
 
 }  // namespace
@@ -1000,6 +1117,24 @@ qmcb_status qmcb_get_positions(qmcb_ctx *ctx, int32_t num, double *R) {
 
 // Rows of the observable series a step range records: one per step that
 // moves electron 0.  Grows the device buffer before the step loop.
+// Drift of the device walk: gradPsiRatio (QF_modification_type none) or,
+// with one determinant, its umrigar93_unequalelectrons modification.
+static bool walk_drift_ok(qmcb_ctx *ctx) {
+  const int dm = ctx->opt.drift_mod;
+  if (dm == QMCB_DRIFT_NONE) return true;
+  if (dm == QMCB_DRIFT_UMRIGAR93_UNEQUAL && !ctx->multi) return true;
+  ctx->err = "device VMC/DMC drift: QF_modification_type none, or umrigar93_unequalelectrons "
+             "with one determinant";
+  return false;
+}
+
+static void walk_propose_args(qmcb_ctx *ctx, ProposeArgs &pa) {
+  pa.drift_mod = ctx->opt.drift_mod;
+  pa.n_nuc = ctx->nn;
+  pa.nuc_xyz = ctx->dsys.nuc_xyz;
+  pa.nuc_z = ctx->dsys.nuc_z;
+}
+
 static qmcb_status series_reserve(qmcb_ctx *ctx, int64_t first_step, int32_t n_steps) {
   const int64_t N = ctx->N;
   const int64_t first = ((first_step + N - 1) / N) * N;  // first multiple of N >= first_step
@@ -1022,10 +1157,7 @@ qmcb_status qmcb_vmc_steps(qmcb_ctx *ctx, int32_t num, int64_t first_step, int32
                            double tau, qmcb_vmc_stats *stats) {
   if (!ctx || num < 1 || num > ctx->Wc || n_steps < 0 || first_step < 0 || !(tau > 0))
     return QMCB_INVALID_ARG;
-  if (ctx->opt.drift_mod != QMCB_DRIFT_NONE) {
-    ctx->err = "device VMC supports QF_modification_type none";
-    return QMCB_INVALID_ARG;
-  }
+  if (!walk_drift_ok(ctx)) return QMCB_INVALID_ARG;
   for (int i = 0; i < num; i++)
     if (!ctx->initialized[i]) {
       ctx->err = "vmc slots must be initialised by an all-electron evaluate";
@@ -1058,6 +1190,7 @@ qmcb_status qmcb_vmc_steps(qmcb_ctx *ctx, int32_t num, int64_t first_step, int32
     pa.Ux = ctx->Ux;
     pa.cgrad = ctx->multi ? ctx->cgrad : nullptr;
     pa.tau = tau;
+    walk_propose_args(ctx, pa);
     pa.seed = ctx->opt.seed;
     pa.step = step;
     pa.propR = ctx->propR;
@@ -1181,8 +1314,9 @@ qmcb_status qmcb_dmc_steps(qmcb_ctx *ctx, int64_t first_step, int32_t n_steps,
     ctx->err = "qmcb_dmc_init first";
     return QMCB_BAD_STATE;
   }
-  if (ctx->opt.drift_mod != QMCB_DRIFT_NONE) {
-    ctx->err = "device DMC supports QF_modification_type none";
+  if (!walk_drift_ok(ctx)) return QMCB_INVALID_ARG;
+  if (prm->energy_modification != 0 && prm->energy_modification != 1) {
+    ctx->err = "energy_modification: 0 (none) or 1 (umrigar93)";
     return QMCB_INVALID_ARG;
   }
   QCHK(hipSetDevice(ctx->device));
@@ -1194,6 +1328,7 @@ qmcb_status qmcb_dmc_steps(qmcb_ctx *ctx, int64_t first_step, int32_t n_steps,
   QCHK(hipMemsetAsync(ctx->acc_w, 0, sizeof(double) * W, ctx->stream));
   if (!ctx->have_e_trial) {
     ctx->e_trial = prm->e_trial;
+    ctx->e_est = prm->e_trial;  // energy_estimated starts at the trial energy
     ctx->have_e_trial = true;
   }
   {
@@ -1225,6 +1360,7 @@ qmcb_status qmcb_dmc_steps(qmcb_ctx *ctx, int64_t first_step, int32_t n_steps,
     pa.Ux = ctx->Ux;
     pa.cgrad = ctx->multi ? ctx->cgrad : nullptr;
     pa.tau = prm->tau;
+    walk_propose_args(ctx, pa);
     pa.seed = ctx->opt.seed;
     pa.step = step;
     pa.propR = ctx->propR;
@@ -1237,6 +1373,8 @@ qmcb_status qmcb_dmc_steps(qmcb_ctx *ctx, int64_t first_step, int32_t n_steps,
     MoveArgs a = DISPATCH(ctx, move_args(ctx, m, 2, none));
     a.tau = prm->tau;
     a.e_trial = ctx->e_trial;
+    a.e_est = ctx->e_est;
+    a.energy_mod = prm->energy_modification;
     a.tau_eff = prm->tau;
     a.reweight = (m == 0 && step >= prm->reweight_after) ? 1 : 0;
     if (m == 0) a.obs = ctx->d_obs;
@@ -1323,7 +1461,8 @@ qmcb_status qmcb_dmc_steps(qmcb_ctx *ctx, int64_t first_step, int32_t n_steps,
     if (sw > 0 && step >= prm->reweight_after) {
       const double ratio = wtot / target;
       const double lr = ratio > 1e-250 ? std::log(ratio) : 0.0;
-      ctx->e_trial = se / sw - prm->pop_control * lr;
+      ctx->e_est = se / sw;
+      ctx->e_trial = ctx->e_est - prm->pop_control * lr;
     }
   }
   // statistics
@@ -1386,6 +1525,20 @@ qmcb_status qmcb_sync(qmcb_ctx *ctx) {
   QCHK(hipSetDevice(ctx->device));
   QCHK(hipStreamSynchronize(ctx->stream));
   tm_collect(ctx);
+  return QMCB_OK;
+}
+
+qmcb_status qmcb_device_bytes(const qmcb_ctx *ctx, int64_t *bytes) {
+  if (!ctx || !bytes) return QMCB_INVALID_ARG;
+  *bytes = (int64_t)ctx->dev_bytes;
+  return QMCB_OK;
+}
+
+qmcb_status qmcb_plan_device_bytes(const qmcb_system *sys, const qmcb_options *opt,
+                                   int64_t *bytes) {
+  if (!sys || !opt || !bytes || opt->max_walkers < 1 || sys->n_basis < 1 || sys->n_nuc < 1)
+    return QMCB_INVALID_ARG;
+  *bytes = (int64_t)plan_device_bytes(sys, opt);
   return QMCB_OK;
 }
 

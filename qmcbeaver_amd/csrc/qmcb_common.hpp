@@ -71,9 +71,9 @@ __device__ __forceinline__ void pade_eval(const qmcb_pade &f, double r, double &
 // ---- wavefront reductions (64 lanes) ---------------------------------------
 // umrigar93_unequalelectrons drift factor of one electron with gradient g at
 // x: nearest nucleus, its charge and the time step (QMCSCFJastrow.cpp:368-443).
-__device__ inline double drift_factor_unequal(const double g[3], const double x[3], int nn,
-                                              const double *nuc_xyz, const double *nuc_z,
-                                              double dt) {
+// The nearest nucleus (first of equals) and the factor given it are split so
+// that callers evaluating two gradients at one position search once.
+__device__ inline int nearest_nucleus(const double x[3], int nn, const double *nuc_xyz) {
   int cn = 0;
   double best = 1e300;
   for (int A = 0; A < nn; A++) {
@@ -82,6 +82,12 @@ __device__ inline double drift_factor_unequal(const double g[3], const double x[
     const double r = sqrt(dx * dx + dy * dy + dz * dz);
     if (r < best) { best = r; cn = A; }
   }
+  return cn;
+}
+
+__device__ inline double drift_factor_near(const double g[3], const double x[3], int cn,
+                                           const double *nuc_xyz, const double *nuc_z,
+                                           double dt) {
   double nv[3], dsq = 0.0;
   for (int k = 0; k < 3; k++) { nv[k] = x[k] - nuc_xyz[3 * cn + k]; dsq += nv[k] * nv[k]; }
   const double inr = 1.0 / sqrt(dsq);
@@ -93,6 +99,12 @@ __device__ inline double drift_factor_unequal(const double g[3], const double x[
   const double t = Z * Z * dsq;
   const double a = 0.5 * (1.0 + dotp) + t / (10.0 * (4.0 + t));
   return (-1.0 + sqrt(1.0 + 2.0 * a * v2 * dt)) / (a * v2 * dt);
+}
+
+__device__ inline double drift_factor_unequal(const double g[3], const double x[3], int nn,
+                                              const double *nuc_xyz, const double *nuc_z,
+                                              double dt) {
+  return drift_factor_near(g, x, nearest_nucleus(x, nn, nuc_xyz), nuc_xyz, nuc_z, dt);
 }
 
 __device__ __forceinline__ double wave_sum(double v) {

@@ -127,6 +127,8 @@ struct MoveArgs {
   double *acc_w;         // [Wc] sum of weights over the steps
   double e_trial, tau_eff;
   int reweight;          // reweight this step (electron 0 moved)
+  int energy_mod;        // 0: energy_modification_type none, 1: umrigar93
+  double e_est;          // energy_estimated of the umrigar93 modification
   // observable sample of this step (null except once per sweep):
   // obs[3 b + {0,1,2}] = {p E_L(R') + (1 - p) E_L(R), weight, p}
   double *obs;
@@ -222,6 +224,11 @@ struct ProposeArgs {
   const double *cgrad;   // several determinants: [Wc][N][3] grad ln D of the sum (else null)
   const double *Ux;
   double tau;
+  // drift of the moved electron: gradPsiRatio (QMCB_DRIFT_NONE) or its
+  // umrigar93_unequalelectrons modification with time step tau
+  int drift_mod;
+  int n_nuc;
+  const double *nuc_xyz, *nuc_z;
   uint64_t seed;
   long step;
   double *propR, *gauss, *unif;

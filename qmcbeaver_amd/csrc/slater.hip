@@ -471,7 +471,7 @@ sm_move_kernel(MoveArgs A, const int *walkers, int num) {
   __shared__ double red[9][SL > 1 ? SL - 1 : 1][NC];
   __shared__ double wv[NC];
   __shared__ double gq[4][NC];
-  __shared__ double wred5[5][NW];
+  __shared__ double wred5[8][NW];
   __shared__ double s_bc[8];
 
   const TO *vr = (const TO *)A.vrow + (long)b * A.v_s_walker;
@@ -607,7 +607,9 @@ sm_move_kernel(MoveArgs A, const int *walkers, int num) {
   const double *gpo = no > 0 ? A.other.gradPR + (long)w * no * 3 : nullptr;
   const double *gps = A.s.gradPR + (long)w * n * 3;
   const double tau = A.tau;
+  const bool qf = A.mode >= 1 && A.drift_mod == QMCB_DRIFT_UMRIGAR93_UNEQUAL;
   double lsum = 0.0, fwd = 0.0, rev = 0.0, lapc = 0.0, gsq = 0.0;
+  double vbt = 0.0, vbo = 0.0, vo = 0.0;  // |modified drift|^2 (trial, current), |v|^2 current
   for (int e = t; e < N; e += NT) {
     double dx[3], dxc[3];
     if (e >= ss && e < ss + n) {
@@ -630,37 +632,62 @@ sm_move_kernel(MoveArgs A, const int *walkers, int num) {
       o[0] = g[0]; o[1] = g[1]; o[2] = g[2];
     }
     if (A.mode >= 1) {
-      // Green's function terms (QMCWalker.cpp:343-398, 756-780); drift =
-      // gradPsiRatio (QF_modification_type none)
+      // Green's function terms (QMCWalker.cpp:343-398, 756-780) with the
+      // drift gradPsiRatio (QF_modification_type none) or its per-electron
+      // umrigar93_unequalelectrons modification (QMCSCFJastrow.cpp:368-443)
+      // at the trial (ft) and current (fo) states
+      double ft = 1.0, fo = 1.0;
+      if (qf) {
+        const double xo[3] = {Rw[3 * e], Rw[3 * e + 1], Rw[3 * e + 2]};
+        const int co = nearest_nucleus(xo, A.n_nuc, A.nuc_xyz);
+        fo = drift_factor_near(gc, xo, co, A.nuc_xyz, A.nuc_z, tau);
+        if (e != A.moved) {
+          ft = drift_factor_near(g, xo, co, A.nuc_xyz, A.nuc_z, tau);
+        } else {
+          const double xt[3] = {A.propR[3L * b], A.propR[3L * b + 1], A.propR[3L * b + 2]};
+          ft = drift_factor_unequal(g, xt, A.n_nuc, A.nuc_xyz, A.nuc_z, tau);
+        }
+        double g2 = 0.0, gc2 = 0.0;
+        for (int k = 0; k < 3; k++) { g2 += g[k] * g[k]; gc2 += gc[k] * gc[k]; }
+        vbt += ft * ft * g2;
+        vbo += fo * fo * gc2;
+        vo += gc2;
+      }
       if (e != A.moved) {
         for (int k = 0; k < 3; k++) {
-          fwd += tau * gc[k] * tau * gc[k];
-          rev += tau * g[k] * tau * g[k];
+          fwd += tau * fo * gc[k] * tau * fo * gc[k];
+          rev += tau * ft * g[k] * tau * ft * g[k];
         }
       } else {
         for (int k = 0; k < 3; k++) {
-          const double tt = Rw[3 * e + k] - A.propR[3L * b + k] - tau * g[k];
+          const double tt = Rw[3 * e + k] - A.propR[3L * b + k] - tau * ft * g[k];
           rev += tt * tt;
           fwd += A.gauss[3L * b + k] * A.gauss[3L * b + k];
         }
       }
     }
   }
-  double vals[5] = {lsum, fwd, rev, lapc, gsq};
+  double vals[8] = {lsum, fwd, rev, lapc, gsq, vbt, vbo, vo};
+  const int nred = qf ? 8 : 5;
 #pragma unroll
-  for (int k = 0; k < 5; k++) {
+  for (int k = 0; k < 8; k++) {
+    if (k >= nred) break;
     const double s = wave_sum(vals[k]);
     if ((t & 63) == 0) wred5[k][t >> 6] = s;
   }
   __syncthreads();
   if (t == 0) {
-    for (int k = 0; k < 5; k++) {
+    for (int k = 0; k < nred; k++) {
       double s = 0.0;
       for (int q = 0; q < NW; q++) s += wred5[k][q];
       s_bc[k] = s;
     }
   }
   __syncthreads();
+  // |modified drift| / |drift| of the trial and current states, for the
+  // umrigar93 energy modification of the DMC weights
+  const double mr_t = (qf && s_bc[4] > 0.0) ? sqrt(s_bc[5]) / sqrt(s_bc[4]) : 1.0;
+  const double mr_o = (qf && s_bc[7] > 0.0) ? sqrt(s_bc[6]) / sqrt(s_bc[7]) : 1.0;
   const double lap_s = s_bc[3];
   const double lap_o = no > 0 ? A.other.lapPR[w] : 0.0;
   const double lapR = lap_s + lap_o + A.pUxx[b] + s_bc[0];
@@ -791,8 +818,18 @@ sm_move_kernel(MoveArgs A, const int *walkers, int num) {
       // (p (S'+S)/2 + (1-p) S)), S = E_T - E_L
       wt = A.weight[w];
       if (A.reweight) {
-        const double so_ = A.e_trial - el_old;
-        const double st_ = (p > 0.0) ? A.e_trial - EL : 0.0;
+        // energy_modification_type none: S = E_T - E_L; umrigar93 (UNR93
+        // eq. 33-34, QMCWalker.cpp:1121-1132): S = (E_T - E_est) +
+        // (E_est - E_L) |modified drift| / |drift|
+        double so_, st_;
+        if (A.energy_mod) {
+          so_ = (A.e_trial - A.e_est) + (A.e_est - el_old) * mr_o;
+          st_ = (A.e_trial - A.e_est) + (A.e_est - EL) * mr_t;
+        } else {
+          so_ = A.e_trial - el_old;
+          st_ = A.e_trial - EL;
+        }
+        if (!(p > 0.0)) st_ = 0.0;
         wt *= exp(A.tau_eff * (p * 0.5 * (so_ + st_) + (1.0 - p) * so_));
         A.weight[w] = wt;
       }

@@ -35,6 +35,13 @@ CASES = [
     ("h2_631gss", dict(walkers=2048, sweeps=6000, tau=0.05, seed=2024, config_seed=778), {}),
     ("be_mcscf", dict(walkers=1024, sweeps=3000, tau=0.02, seed=2025, config_seed=779),
      {"replace_electron_nucleus_cusps": "1"}),
+    # the reference's default drift (umrigar93_unequalelectrons, time step
+    # dt = tau) at a large time step, where the Green's function ratio
+    # decides the acceptance; the reference starts a run with dt set to
+    # dt_equilibration (QMCFlags.cpp:916), so both are set
+    ("h2_631gss", dict(walkers=2048, sweeps=4000, tau=0.3, seed=2026, config_seed=780),
+     {"QF_modification_type": "umrigar93_unequalelectrons", "dt": "0.3",
+      "dt_equilibration": "0.3"}, "h2_631gss_qf"),
 ]
 
 
@@ -42,12 +49,14 @@ def main():
     if not os.path.exists(DRIVER):
         sys.exit("build the reference driver first: make -f oracle/ref.mk")
     only = set(sys.argv[1:])  # optional: names of the cases to (re)write
-    for name, run, extra in CASES:
-        if not only or name in only:
-            one(name, run, dict(OVERRIDES, **extra))
+    for case in CASES:
+        name, run, extra = case[:3]
+        tag = case[3] if len(case) > 3 else name
+        if not only or tag in only:
+            one(name, run, dict(OVERRIDES, **extra), tag)
 
 
-def one(name, RUN, overrides):
+def one(name, RUN, overrides, tag):
     with tempfile.TemporaryDirectory() as tmp:
         raw = os.path.join(tmp, "in_raw.ckmf")
         with gzip.open(os.path.join(OUT, name + ".ckmf.gz"), "rb") as g, open(raw, "wb") as f:
@@ -67,7 +76,7 @@ def one(name, RUN, overrides):
         res = json.loads(out.strip().splitlines()[-1])
     res.pop("seconds", None)
     rec = dict(RUN, input=name, overrides=overrides, reference=res)
-    with open(os.path.join(OUT, f"vmc_{name}.json"), "w") as f:
+    with open(os.path.join(OUT, f"vmc_{tag}.json"), "w") as f:
         json.dump(rec, f, indent=1)
     print(rec)
 

@@ -12,6 +12,7 @@ import numpy as np
 import pytest
 
 from qmcbeaver_amd import synth, Context
+from qmcbeaver_amd._abi import QMCB_DRIFT_NONE, QMCB_DRIFT_UMRIGAR93_UNEQUAL
 from qmcbeaver_amd.stats import Property, series_energy
 from tests.inputs_util import configs, load
 
@@ -53,7 +54,8 @@ def test_dmc_branching_keeps_walker_states_consistent():
             ctx.accept(slots, np.zeros(len(slots), dtype=np.uint8))
 
 
-def test_dmc_h2_reaches_exact_ground_state():
+@pytest.mark.parametrize("modified", [False, True])
+def test_dmc_h2_reaches_exact_ground_state(modified):
     """The reference's H2 6-31G** example: HF orbital (sigma_g, no node), Pade
     Jastrow and the reference's default electron-nucleus cusp replacement.
     The DMC energy, averaged with the DDDA estimator over the per-sweep
@@ -61,24 +63,29 @@ def test_dmc_h2_reaches_exact_ground_state():
     E0 = -1.174476 Ha (the reference's test/README), within 4 sigma or
     1.5 mHa; the VMC energy of the same trial function lies above it.
     (Without the cusp replacement the Gaussian basis's wrong cusp leaves a
-    time-step bias of about -2 mHa at tau = 0.01.)"""
+    time-step bias of about -2 mHa at tau = 0.01.)  Run with plain drift and
+    weights, and with the reference's defaults: umrigar93_unequalelectrons
+    drift and umrigar93 energy modification."""
     packed, _, _ = load("h2_631gss")
     W = 2048
-    print("case dmc H2 6-31G** exact ground state", flush=True)
+    print("case dmc H2 6-31G** exact ground state, modified", modified, flush=True)
     R0 = configs(packed, W, 42)
     N = R0.shape[1]
+    drift = QMCB_DRIFT_UMRIGAR93_UNEQUAL if modified else QMCB_DRIFT_NONE
     with Context(packed, precision="fp64", max_walkers=4 * W, seed=11,
-                 replace_cusps=True) as ctx:
+                 replace_cusps=True, drift_mod=drift, dt=0.01) as ctx:
         ctx.evaluate(np.arange(W), R0)
         vmc = ctx.vmc_steps(W, 0, 400 * N, 0.05)
         e_vmc = vmc.sum_e_local / vmc.n_samples
         ctx.dmc_init(W)
         tau = 0.01
-        st = ctx.dmc_steps(0, 2000 * N, tau, e_vmc, target=W, reweight_after=0)  # equilibration
+        st = ctx.dmc_steps(0, 2000 * N, tau, e_vmc, target=W, reweight_after=0,
+                           energy_modification=modified)  # equilibration
         p = Property()
         step = 2000 * N
         for blk in range(16):
-            st = ctx.dmc_steps(step, 1000 * N, tau, st.e_trial, target=W, reweight_after=0)
+            st = ctx.dmc_steps(step, 1000 * N, tau, st.e_trial, target=W, reweight_after=0,
+                               energy_modification=modified)
             step += 1000 * N
             rows = ctx.sample_series()
             assert len(rows) == 1000

@@ -1,0 +1,129 @@
+"""Many molecules on one device: footprint projection, isolation of jobs
+in the mix, and the serving loop's admission with real contexts.
+
+* qmcb_plan_device_bytes equals what a context holds after its first
+  single-electron evaluate (single determinant, every precision, several
+  determinants), plus 32 bytes per recorded sweep of the series, and
+  bounds it with cusp replacement (cusp tables projected at their maximum);
+* isolation: every job's per-block results (VMC energy and acceptance sums,
+  DMC weighted energy sums, populations, trial energies) are bit-identical
+  when the job runs alone and when it runs in the mix with others on the
+  device at the same time -- the random streams are counter-based and no
+  reduction crosses contexts, so the stated tolerance is exact equality;
+* batched DMC: the same molecules advanced by 1 host thread and by 6 give
+  identical per-molecule results;
+* the serving loop with a capacity that forces queueing never holds more
+  projected bytes than the capacity, and every job's context stayed within
+  its projection.
+"""
+import os
+import tempfile
+import gzip
+
+import numpy as np
+import pytest
+
+from qmcbeaver_amd import Context, ckmf, serving, synth
+from qmcbeaver_amd.core import plan_device_bytes
+from tests.inputs_util import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+
+def _mcscf(name):
+    with tempfile.TemporaryDirectory() as tmp:
+        raw = os.path.join(tmp, name + ".ckmf")
+        with gzip.open(os.path.join(GOLDEN, name + ".ckmf.gz"), "rb") as g, open(raw, "wb") as f:
+            f.write(g.read())
+        packed, _, _ = ckmf.read_ckmf(raw)
+    return packed
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64", "fp32store"])
+@pytest.mark.parametrize("n_elec,W,scratch", [(16, 100, 0), (64, 300, 1 << 20)])
+def test_plan_equals_allocation(precision, n_elec, W, scratch):
+    p = synth.make_system(-1, seed=11 + n_elec, n_elec=n_elec).pack()
+    plan = plan_device_bytes(p, precision=precision, max_walkers=W, basis_scratch_bytes=scratch)
+    s = synth.make_system(-1, seed=11 + n_elec, n_elec=n_elec)
+    R = synth.initial_walkers(s, W, 3)
+    with Context(p, precision=precision, max_walkers=W, basis_scratch_bytes=scratch) as c:
+        c.evaluate(np.arange(W), R)
+        before = c.device_bytes()
+        R[:, 0] += 0.1
+        c.evaluate(np.arange(W), R, whichE=0)  # allocates the staging arrays
+        assert before < c.device_bytes() == plan
+        c.vmc_steps(W, 0, 2, 0.1)  # + one row of the per-sweep series (32 bytes)
+        assert c.device_bytes() == plan + 32
+
+
+def test_plan_multidet_and_cusps():
+    p = _mcscf("be_mcscf")
+    W = 64
+    R = np.random.default_rng(1).normal(size=(W, int(p["n_up"]) + int(p["n_dn"]), 3))
+    plan = plan_device_bytes(p, precision="fp64", max_walkers=W)
+    with Context(p, precision="fp64", max_walkers=W) as c:
+        c.evaluate(np.arange(W), R)
+        c.vmc_steps(W, 0, 2, 0.1)  # several determinants stage moves; one series row
+        assert c.device_bytes() == plan + 32
+    plan_c = plan_device_bytes(p, precision="fp64", max_walkers=W, replace_cusps=True)
+    with Context(p, precision="fp64", max_walkers=W, replace_cusps=True) as c:
+        c.evaluate(np.arange(W), R)
+        c.vmc_steps(W, 0, 2, 0.1)
+        assert plan <= c.device_bytes() <= plan_c
+
+
+def _small_trace():
+    jobs = serving.poisson_trace(8, 200.0, seed=21, e_range=(8, 48), moves_range=(2e3, 2e4),
+                                 block_steps=40, max_blocks=3, dmc_fraction=0.4)
+    assert any(j.kind == "dmc" for j in jobs) and any(j.kind == "vmc" for j in jobs)
+    return jobs
+
+
+def _fresh(jobs):
+    return [serving.Job(job_id=j.job_id, n_elec=j.n_elec, walkers=j.walkers, blocks=j.blocks,
+                        kind=j.kind, block_steps=j.block_steps, seed=j.seed, arrival=j.arrival)
+            for j in jobs]
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_job_results_same_alone_and_in_the_mix(precision):
+    jobs = _small_trace()
+    runner = serving.JobRunner(precision=precision)
+    alone = serving.run_alone(_fresh(jobs), runner)
+    mixed = _fresh(jobs)
+    srv = serving.Server(capacity_bytes=64 << 30, workers=4, runner=runner)
+    out = srv.run(mixed)
+    assert out["completed"] == len(jobs), out["failed"]
+    assert srv.peak_active >= 2  # the jobs did overlap
+    for j in mixed:
+        assert len(j.results) == j.blocks
+        assert j.results == alone[j.job_id], j.job_id
+        assert 0 < j.device_bytes <= j.footprint
+
+
+def test_batched_dmc_thread_independent():
+    comp = [(ne, W, sd) for ne, W, sd in serving.batched_composition(6, seed=5, e_range=(8, 40),
+                                                                     w_range=(64, 256))]
+    res = []
+    for workers in (1, 6):
+        rows = [None] * len(comp)
+        with serving.BatchedDMC(comp, precision="fp32", workers=workers) as b:
+            b.setup()
+            b.advance(30)
+            b.advance(30, rows)
+        res.append(rows)
+    assert res[0] == res[1]
+    for sw_e, sw, pop, e_t in res[0]:
+        assert sw > 0 and pop > 0 and np.isfinite(sw_e / sw) and np.isfinite(e_t)
+
+
+def test_server_capacity_forces_queueing():
+    jobs = _small_trace()
+    runner = serving.JobRunner(precision="fp32")
+    fps = sorted(runner.footprint(j) for j in jobs)
+    cap = fps[-1] + fps[-2] // 2  # at most one large job at a time
+    srv = serving.Server(capacity_bytes=cap, workers=4, runner=runner)
+    out = srv.run(_fresh(jobs))
+    assert out["completed"] == len(jobs), out["failed"]
+    assert srv.peak_bytes <= cap
+    assert out["block_latency_ms"]["p99"] >= out["block_latency_ms"]["p50"] > 0

@@ -16,6 +16,7 @@ import numpy as np
 import pytest
 
 from qmcbeaver_amd import Context
+from qmcbeaver_amd._abi import QMCB_DRIFT_UMRIGAR93_UNEQUAL
 from qmcbeaver_amd.stats import Property, series_energy
 from tests.inputs_util import GOLDEN, configs, load
 
@@ -66,3 +67,38 @@ def test_series_rows_follow_sweeps_across_calls():
             counts.append(len(ctx.sample_series()))
         # steps 0 | 1-4 (2, 4) | 5-7 (6) | none | 8-14 (8, 10, 12, 14)
         assert counts == [1, 2, 1, 0, 4]
+
+
+def test_vmc_modified_drift_matches_reference():
+    """The reference's default drift, umrigar93_unequalelectrons, at tau = 0.3
+    (reference run: tests/golden/vmc_h2_631gss_qf.json).  At this time step
+    the acceptance depends on the drift in the proposal and in the Green's
+    function ratio: over these 1.6e7 moves the reference accepts 0.92335
+    with the plain drift and 0.92125 with the modified one (binomial error
+    about 7e-5).  The device acceptance must match the reference's within
+    5e-4, and the energy within 4 combined error bars."""
+    with open(os.path.join(GOLDEN, "vmc_h2_631gss_qf.json")) as f:
+        g = json.load(f)
+    packed, _, _ = load(g["input"])
+    W, sweeps, tau = g["walkers"], g["sweeps"], g["tau"]
+    R0 = configs(packed, W, g["config_seed"])
+    N = R0.shape[1]
+    print("case vmc modified drift H2 6-31G**, tau", tau, flush=True)
+    with Context(packed, precision="fp64", max_walkers=W, seed=98,
+                 drift_mod=QMCB_DRIFT_UMRIGAR93_UNEQUAL, dt=tau) as ctx:
+        ctx.evaluate(np.arange(W), R0)
+        p = Property()
+        acc = 0.0
+        for blk in range(4):
+            n = sweeps // 4
+            st = ctx.vmc_steps(W, blk * n * N, n * N, tau)
+            acc += st.sum_accept
+            series_energy(ctx.sample_series(), p)
+    s = p.summary()
+    ref = g["reference"]
+    acc /= W * sweeps * N
+    print("device", s.average, "+-", s.std_dev, "acceptance", acc, "reference",
+          ref["ddda_energy"], "+-", ref["ddda_std"], ref["acceptance"], flush=True)
+    assert abs(acc - ref["acceptance"]) < 5e-4
+    comb = np.hypot(s.std_dev, ref["ddda_std"])
+    assert abs(s.average - ref["ddda_energy"]) < 4 * comb
