@@ -134,7 +134,7 @@ def accuracy_report(s, n_conf=10000, seed=4321, tau=0.01, sweeps=1, precision="f
         d = np.abs(np.asarray(x)[ok] - np.asarray(y)[ok]).reshape(ok.sum(), -1)
         d = d.max(axis=1) / per
         return {"max": float(d.max()), "rms": float(np.sqrt(np.mean(d * d))),
-                "p99": float(np.percentile(d, 99))}
+                "p99": float(np.percentile(d, 99)), "median": float(np.median(d))}
     out = {"configurations": int(ok.sum()), "reference": "FP64 path", "precision": precision,
            "all_electron": {"log_psi": stats(a32["log_psi"], a64["log_psi"]),
                             "grad_lnpsi": stats(a32["grad_lnpsi"], a64["grad_lnpsi"]),
@@ -233,7 +233,7 @@ def main():
     ap.add_argument("--tau", type=float, default=0.01)
     ap.add_argument("--block", type=int, default=16, help="steps per timed block")
     ap.add_argument("--refresh", type=int, default=64)
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp32store", "fp64"])
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp32store", "bf16", "fp64"])
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--accuracy", type=int, default=10000,
@@ -314,12 +314,15 @@ def main():
                      args.block, args.refresh if args.precision != "fp64" else 0, 1234)
     fp64 = None
     store = None
+    bf16 = None
     if not args.no_fp64 and args.precision != "fp64":
         fp64 = run_path(s, "fp64", args.walkers, args.steps, args.warmup, args.tau, args.block,
                         0, 1234)
         if args.precision == "fp32":
             store = run_path(s, "fp32store", args.walkers, args.steps, args.warmup, args.tau,
                              args.block, args.refresh, 1234)
+            bf16 = run_path(s, "bf16", args.walkers, args.steps, args.warmup, args.tau,
+                            args.block, args.refresh, 1234)
 
     s_d = 8 if args.precision == "fp64" else 4
     kt = mixed["kernels"].get("sm_move", {"ms": float("nan"), "launches": 1})
@@ -383,12 +386,25 @@ def main():
             "over_fp64": store["walker_moves_per_s"] / fp64["walker_moves_per_s"],
             "kernels_ms_per_step": {k: v["ms"] / max(args.steps, 1)
                                     for k, v in store["kernels"].items()}}
+    if bf16 is not None:
+        out["bf16"] = {
+            "description": "FP64 basis rounded to BF16, BF16 coefficients, "
+                           "v_mfma_f32_16x16x16_bf16 with FP32 accumulate, FP32 derivative "
+                           "storage, FP64 inverse/log-det/E_L",
+            "value": bf16["walker_moves_per_s"], "ms_per_step": bf16["ms_per_step"],
+            "over_fp64": None if fp64 is None else
+            bf16["walker_moves_per_s"] / fp64["walker_moves_per_s"],
+            "kernels_ms_per_step": {k: v["ms"] / max(args.steps, 1)
+                                    for k, v in bf16["kernels"].items()}}
     if args.accuracy and args.precision != "fp64":
         out["mixed_error"] = accuracy_report(s, args.accuracy, tau=args.tau,
                                              precision=args.precision)
         if store is not None:
             out["fp32store"]["error"] = accuracy_report(s, args.accuracy, tau=args.tau,
                                                         precision="fp32store")
+        if bf16 is not None:
+            out["bf16"]["error"] = accuracy_report(s, args.accuracy, tau=args.tau,
+                                                   precision="bf16")
     if args.serving_jobs > 0:
         srv = serving_run(args.precision, args.serving_jobs, args.rate, 20, args.serving_workers,
                           args.capacity_gib)

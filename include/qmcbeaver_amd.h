@@ -41,7 +41,9 @@ typedef enum qmcb_precision {
   QMCB_MIXED_FP32 = 1,  /* orbital GEMM on FP32 MFMA with FP32 operands and
                            accumulate (basis computed in FP64, rounded); FP32
                            derivative storage; FP64 re-inversion                  */
-  QMCB_MIXED_BF16 = 2,  /* reserved: GEMM operands BF16, FP32 accumulate         */
+  QMCB_MIXED_BF16 = 2,  /* as MIXED_FP32 with BF16 GEMM operands (basis values
+                           and coefficients rounded from FP64) on
+                           v_mfma_f32_16x16x16_bf16, FP32 accumulate            */
   QMCB_MIXED_F32STORE = 3 /* FP64 basis and GEMM, derivative matrices stored
                            in FP32 (the HBM-bound part halves)                    */
 } qmcb_precision;

@@ -37,6 +37,12 @@ template <>
 __device__ __forceinline__ __hip_bfloat16 to_store<float, __hip_bfloat16>(float v) {
   return __float2bfloat16(v);
 }
+// FP64 basis values rounded to FP32, then to BF16 (both round-to-nearest-
+// even), as the host rounds the coefficients
+template <>
+__device__ __forceinline__ __hip_bfloat16 to_store<double, __hip_bfloat16>(double v) {
+  return __float2bfloat16((float)v);
+}
 
 // X layout: [5][ldk][ldp]: component, basis function (padded rows zero),
 // point (padded columns untouched).
@@ -142,5 +148,8 @@ template hipError_t launch_basis<float, float>(const DevSystem &, const PointSet
 template hipError_t launch_basis<float, __hip_bfloat16>(const DevSystem &, const PointSet &,
                                                         __hip_bfloat16 *, int, long,
                                                         hipStream_t);
+template hipError_t launch_basis<double, __hip_bfloat16>(const DevSystem &, const PointSet &,
+                                                         __hip_bfloat16 *, int, long,
+                                                         hipStream_t);
 
 }  // namespace qmcb

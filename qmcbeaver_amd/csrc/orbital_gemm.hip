@@ -10,11 +10,16 @@
 // Ct = C^T ([basis][orbital], zero padded to 16-multiples of orbitals and
 // 32-multiples of basis functions), X from kernel (a).  One wavefront owns a
 // 16-point tile and MT 16-orbital tiles; each k-step issues MT MFMA
-// 16x16x4 instructions (FP64: v_mfma_f64_16x16x4f64, FP32:
-// v_mfma_f32_16x16x4f32) with lane l supplying A[l%16][l/16] and
-// B[l/16][l%16].  Accumulator element i (i = 0..3) of lane l is
-// D[row][l%16] with row = 4(l/16) + i for the FP32 instruction and
+// instructions: 16x16x4 for FP64 (v_mfma_f64_16x16x4f64) and FP32
+// (v_mfma_f32_16x16x4f32) with lane l supplying A[l%16][l/16] and
+// B[l/16][l%16]; 16x16x16 for BF16 operands (v_mfma_f32_16x16x16_bf16,
+// FP32 accumulate) with lane l supplying the four k values 4(l/16)..+3 of
+// A[l%16][.] and B[.][l%16] (any k assignment shared by A and B gives the
+// same sum).  Accumulator element i (i = 0..3) of lane l is D[row][l%16]
+// with row = 4(l/16) + i for the FP32-output instructions and
 // row = 4i + l/16 for the FP64 one (Mfma<T>::row).
+#include <hip/hip_bf16.h>
+
 #include "qmcb_kernels.hpp"
 
 namespace qmcb {
@@ -30,7 +35,7 @@ template <>
 struct Mfma<double> {
   typedef d4v acc_t;
   typedef double acc_s;
-  static constexpr int KSTEP = 4;
+  static constexpr int KPL = 1;  // k values per lane per instruction
   __device__ static acc_t run(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
@@ -42,9 +47,20 @@ template <>
 struct Mfma<float> {
   typedef f4v acc_t;
   typedef float acc_s;
-  static constexpr int KSTEP = 4;
+  static constexpr int KPL = 1;
   __device__ static acc_t run(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  __device__ static constexpr int row(int lk, int i) { return 4 * lk + i; }
+};
+
+template <>
+struct Mfma<__hip_bfloat16> {
+  typedef f4v acc_t;
+  typedef float acc_s;
+  static constexpr int KPL = 4;
+  __device__ static acc_t run(s4v a, s4v b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
   }
   __device__ static constexpr int row(int lk, int i) { return 4 * lk + i; }
 };
@@ -58,7 +74,7 @@ struct Mfma<float> {
 template <typename TI>
 struct GemmTile {
   static constexpr int KC = 32;                       // k rows per chunk
-  static constexpr int PAD = sizeof(TI) == 4 ? 16 : 8;  // elements
+  static constexpr int PAD = sizeof(TI) == 8 ? 8 : 16;  // elements
 };
 
 template <typename TI, typename TO, int MT>
@@ -102,7 +118,8 @@ orbital_gemm_kernel(const TI *__restrict__ Ct, int ldc, int n_orb, const TI *__r
       if (e < KC * OW) *(u4 *)&cs[buf][r][c] = stage[j];
     }
   };
-  const TI *__restrict__ xp = X + (long)comp * ldk * ldp + (long)lk * ldp + p0 + li;
+  constexpr int KPL = M::KPL, KS = 4 * KPL;  // k values per lane, per instruction
+  const TI *__restrict__ xp = X + (long)comp * ldk * ldp + (long)(KPL * lk) * ldp + p0 + li;
   const int nch = kpad / KC;
   // the point operand of the next chunk is loaded with the next coefficient
   // chunk, one chunk ahead of its MFMAs (its HBM latency overlaps them)
@@ -110,7 +127,9 @@ orbital_gemm_kernel(const TI *__restrict__ Ct, int ldc, int n_orb, const TI *__r
   auto bload = [&](int k0, TI *dst) {
     if (!live) return;
 #pragma unroll
-    for (int s = 0; s < KC / 4; s++) dst[s] = xp[(long)(k0 + 4 * s) * ldp];
+    for (int s = 0; s < KC / KS; s++)
+#pragma unroll
+      for (int j = 0; j < KPL; j++) dst[s * KPL + j] = xp[(long)(k0 + KS * s + j) * ldp];
   };
   gload(0);
   bload(0, b);
@@ -123,13 +142,30 @@ orbital_gemm_kernel(const TI *__restrict__ Ct, int ldc, int n_orb, const TI *__r
       bload(k0 + KC, bn);
     }
     if (live) {
+      if constexpr (KPL == 1) {
 #pragma unroll
-      for (int s = 0; s < KC / 4; s++) {
-        TI a[MT];
+        for (int s = 0; s < KC / 4; s++) {
+          TI a[MT];
 #pragma unroll
-        for (int t = 0; t < MT; t++) a[t] = cs[buf][4 * s + lk][16 * t + li];
+          for (int t = 0; t < MT; t++) a[t] = cs[buf][4 * s + lk][16 * t + li];
 #pragma unroll
-        for (int t = 0; t < MT; t++) acc[t] = M::run(a[t], b[s], acc[t]);
+          for (int t = 0; t < MT; t++) acc[t] = M::run(a[t], b[s], acc[t]);
+        }
+      } else {
+#pragma unroll
+        for (int s = 0; s < KC / KS; s++) {
+          s4v bv;
+#pragma unroll
+          for (int j = 0; j < KPL; j++) bv[j] = __builtin_bit_cast(short, b[s * KPL + j]);
+#pragma unroll
+          for (int t = 0; t < MT; t++) {
+            s4v av;
+#pragma unroll
+            for (int j = 0; j < KPL; j++)
+              av[j] = __builtin_bit_cast(short, cs[buf][KS * s + KPL * lk + j][16 * t + li]);
+            acc[t] = M::run(av, bv, acc[t]);
+          }
+        }
       }
     }
     if (ch + 1 < nch) {
@@ -223,5 +259,9 @@ template hipError_t launch_orbital_gemm<float, float>(const float *, int, int, c
 template hipError_t launch_orbital_gemm<double, float>(const double *, int, int, const double *,
                                                        int, long, long, int, const GemmOut &,
                                                        hipStream_t);
+template hipError_t launch_orbital_gemm<__hip_bfloat16, float>(const __hip_bfloat16 *, int, int,
+                                                               const __hip_bfloat16 *, int, long,
+                                                               long, int, const GemmOut &,
+                                                               hipStream_t);
 
 }  // namespace qmcb

@@ -3,6 +3,7 @@
 // A context owns one molecule and the device-resident state of up to
 // max_walkers walkers.  Every entry point runs on the context's stream; the
 // host-buffer entry points copy in, launch, copy out and synchronise.
+#include <hip/hip_bf16.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -154,6 +155,15 @@ hipError_t dalloc_bytes(qmcb_ctx *ctx, void **p, size_t bytes) {
   ctx->allocs.push_back(*p);
   ctx->dev_bytes += bytes;
   return hipMemsetAsync(*p, 0, bytes, ctx->stream);
+}
+
+// FP32 -> BF16 bits, round to nearest even (NaN kept quiet)
+uint16_t bf16_bits(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
 }
 
 int pad_orb(int n) {
@@ -454,11 +464,14 @@ struct Engine {
 typedef Engine<double, double, double, double> EngineF64;
 typedef Engine<float, float, double, float> EngineF32;       // FP64 basis, FP32 MFMA
 typedef Engine<float, double, double, double> EngineF32Store;  // FP64 GEMM, FP32 storage
+typedef Engine<float, __hip_bfloat16, double, float> EngineBF16;  // FP64 basis, BF16 MFMA
 
 #define DISPATCH(ctx, call)                                                          \
   ((ctx)->opt.precision == QMCB_FP64                                                 \
        ? EngineF64::call                                                             \
-       : ((ctx)->opt.precision == QMCB_MIXED_F32STORE ? EngineF32Store::call : EngineF32::call))
+       : ((ctx)->opt.precision == QMCB_MIXED_F32STORE                                \
+              ? EngineF32Store::call                                                 \
+              : ((ctx)->opt.precision == QMCB_MIXED_BF16 ? EngineBF16::call : EngineF32::call)))
 
 qmcb_status check_ids(qmcb_ctx *ctx, int num, const int32_t *ids) {
   if (num < 0 || num > ctx->Wc) {
@@ -540,6 +553,7 @@ qmcb_status create_impl(const qmcb_system *s, const qmcb_options *o, qmcb_ctx *c
   const bool store32 = o->precision == QMCB_MIXED_F32STORE;
   ctx->ts = f64 ? 8 : 4;
   ctx->tx = ctx->to = (f64 || store32) ? 8 : 4;
+  if (o->precision == QMCB_MIXED_BF16) ctx->tx = 2;  // BF16 GEMM operands, FP32 output
   QCHK(hipSetDevice(o->device));
   ctx->device = o->device;
   QCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
@@ -648,6 +662,11 @@ qmcb_status create_impl(const qmcb_system *s, const qmcb_options *o, qmcb_ctx *c
     QCHK(dalloc_bytes(ctx, &ctx->Ct[sp], cnt * ctx->tx));
     if (ctx->tx == 8) {
       QCHK(hipMemcpyAsync(ctx->Ct[sp], ct.data(), cnt * 8, hipMemcpyHostToDevice, ctx->stream));
+      QCHK(hipStreamSynchronize(ctx->stream));
+    } else if (ctx->tx == 2) {
+      std::vector<uint16_t> cb(cnt);
+      for (size_t i = 0; i < cnt; i++) cb[i] = bf16_bits((float)ct[i]);
+      QCHK(hipMemcpyAsync(ctx->Ct[sp], cb.data(), cnt * 2, hipMemcpyHostToDevice, ctx->stream));
       QCHK(hipStreamSynchronize(ctx->stream));
     } else {
       std::vector<float> cf(cnt);
@@ -837,7 +856,8 @@ size_t plan_device_bytes(const qmcb_system *s, const qmcb_options *o) {
   auto add = [&](size_t count, size_t elem) { total += std::max<size_t>(count * elem, 16); };
   const bool f64 = o->precision == QMCB_FP64;
   const bool store32 = o->precision == QMCB_MIXED_F32STORE;
-  const size_t ts = f64 ? 8 : 4, tx = (f64 || store32) ? 8 : 4, to = tx;
+  const size_t ts = f64 ? 8 : 4, to = (f64 || store32) ? 8 : 4;
+  const size_t tx = o->precision == QMCB_MIXED_BF16 ? 2 : to;
   const long W = o->max_walkers, N = s->n_up + s->n_dn, nn = s->n_nuc, B = s->n_basis;
   const long kpad = ((B + 31) / 32) * 32;
   const long n[2] = {s->n_up, s->n_dn};
@@ -962,7 +982,7 @@ qmcb_status qmcb_create(const qmcb_system *sys, const qmcb_options *opt, qmcb_ct
     return QMCB_INVALID_ARG;
   }
   if (opt->precision != QMCB_FP64 && opt->precision != QMCB_MIXED_FP32 &&
-      opt->precision != QMCB_MIXED_F32STORE) {
+      opt->precision != QMCB_MIXED_F32STORE && opt->precision != QMCB_MIXED_BF16) {
     g_create_error = "precision not supported by this build";
     return QMCB_INVALID_ARG;
   }

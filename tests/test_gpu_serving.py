@@ -39,7 +39,7 @@ def _mcscf(name):
     return packed
 
 
-@pytest.mark.parametrize("precision", ["fp32", "fp64", "fp32store"])
+@pytest.mark.parametrize("precision", ["fp32", "fp64", "fp32store", "bf16"])
 @pytest.mark.parametrize("n_elec,W,scratch", [(16, 100, 0), (64, 300, 1 << 20)])
 def test_plan_equals_allocation(precision, n_elec, W, scratch):
     p = synth.make_system(-1, seed=11 + n_elec, n_elec=n_elec).pack()

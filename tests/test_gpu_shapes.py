@@ -10,7 +10,11 @@ single-electron proposals (the row GEMM and its LDS-transposed epilogue).
 Bars: FP64 1e-10 relative (1e-9 after Sherman-Morrison updates); mixed
 modes within their stated envelopes (FP32 store: E_L 1e-6 Ha/electron;
 FP32 operands: ln|Psi| 0.05, median E_L 1e-3 Ha/electron -- single
-ill-conditioned configurations can be far worse in that mode).
+ill-conditioned configurations can be far worse in that mode).  BF16
+operands (the third MFMA instruction, 16x16x16) carry 8-bit mantissas, so
+on these unequilibrated configurations ln|Psi| moves by up to ~2 and E_L by
+up to tens of Ha/electron; the pin there is the mapping itself: no
+singular matrix, finite results, ln|Psi| within 5.
 """
 import sys
 
@@ -80,7 +84,7 @@ def test_single_electron_partial_tiles(system):
             ctx.accept(ids, np.ones(W, dtype=np.uint8))
 
 
-@pytest.mark.parametrize("precision", ["fp32store", "fp32"])
+@pytest.mark.parametrize("precision", ["fp32store", "fp32", "bf16"])
 def test_mixed_partial_tiles(system, precision):
     s, pk = system
     W = 3
@@ -95,8 +99,10 @@ def test_mixed_partial_tiles(system, precision):
         dl = np.abs(res["log_psi"] - ref["log_psi"])
         if precision == "fp32store":
             assert el.max() < 1e-6 and dl.max() < 1e-9, (el.max(), dl.max())
-        else:
+        elif precision == "fp32":
             assert np.median(el) < 1e-3 and dl.max() < 0.05, (el, dl.max())
+        else:
+            assert np.isfinite(res["e_local"]).all() and dl.max() < 5.0, (el, dl)
         # one single-electron move through the FP32/FP64 row GEMM epilogue
         e = N - 1
         R2 = R.copy()
@@ -106,5 +112,8 @@ def test_mixed_partial_tiles(system, precision):
     el2 = np.abs(res2["e_local"] - ref2["e_local"]) / N
     if precision == "fp32store":
         assert el2.max() < 1e-6, el2.max()
-    else:
+    elif precision == "fp32":
         assert np.median(el2) < 1e-3, el2
+    else:
+        assert not res2["singular"].any() and np.isfinite(res2["e_local"]).all()
+        assert np.abs(res2["log_psi"] - ref2["log_psi"]).max() < 5.0

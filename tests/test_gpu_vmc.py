@@ -30,7 +30,8 @@ def say(*a):
     sys.stdout.flush()
 
 
-@pytest.mark.parametrize("precision,refresh,tol", [("fp64", 0, 1e-8), ("fp32", 64, 1e-6)])
+@pytest.mark.parametrize("precision,refresh,tol", [("fp64", 0, 1e-8), ("fp32", 64, 1e-6),
+                                                    ("bf16", 64, 1e-6)])
 def test_vmc_resident_state_matches_fresh_evaluate(precision, refresh, tol):
     s = synth.make_system(1)
     W, steps = 64, 300

@@ -24,8 +24,8 @@ def test_plan_grows_with_walkers_and_precision():
     s = synth.make_system(-1, seed=5, n_elec=40)
     p = s.pack()
     b = {prec: plan_device_bytes(p, precision=prec, max_walkers=512)
-         for prec in ("fp32", "fp32store", "fp64")}
-    assert b["fp32"] < b["fp32store"] < b["fp64"]
+         for prec in ("bf16", "fp32", "fp32store", "fp64")}
+    assert b["bf16"] < b["fp32"] < b["fp32store"] < b["fp64"]
     small = plan_device_bytes(p, precision="fp32", max_walkers=256)
     big = plan_device_bytes(p, precision="fp32", max_walkers=4096)
     # per-walker resident state: at least inverse + stored orbital rows
