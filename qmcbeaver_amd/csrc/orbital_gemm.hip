@@ -65,8 +65,8 @@ struct Mfma<__hip_bfloat16> {
   __device__ static constexpr int row(int lk, int i) { return 4 * lk + i; }
 };
 
-// Coefficient k-chunks are staged in LDS once per workgroup (4 waves share
-// them) and double-buffered: while the waves run the MFMAs of chunk c from
+// Coefficient k-chunks are staged in LDS once per workgroup (WPB waves, 2 or
+// 4, share them) and double-buffered: while the waves run the MFMAs of chunk c from
 // LDS, the next chunk is loaded from L2 into registers.  The point operand
 // X streams from HBM exactly once.  LDS rows are padded so that the four
 // k-rows one MFMA operand read touches (lanes 0-15 / 16-31 / ...) fall in
@@ -77,8 +77,8 @@ struct GemmTile {
   static constexpr int PAD = sizeof(TI) == 8 ? 8 : 16;  // elements
 };
 
-template <typename TI, typename TO, int MT>
-__global__ void __launch_bounds__(256)
+template <typename TI, typename TO, int MT, int WPB>
+__global__ void __launch_bounds__(64 * WPB)
 orbital_gemm_kernel(const TI *__restrict__ Ct, int ldc, int n_orb, const TI *__restrict__ X,
                     int ldk, long ldp, long P, int kpad, GemmOut o) {
   typedef Mfma<TI> M;
@@ -87,12 +87,13 @@ orbital_gemm_kernel(const TI *__restrict__ Ct, int ldc, int n_orb, const TI *__r
   constexpr int OW = 16 * MT;                    // orbital columns of the tile
   constexpr int LW = OW + GemmTile<TI>::PAD;     // LDS row length
   constexpr int VE = 16 / sizeof(TI);            // elements per 16-byte load
-  constexpr int NV = (KC * OW + 256 * VE - 1) / (256 * VE);  // 16-byte loads per thread, chunk
+  constexpr int NT = 64 * WPB;
+  constexpr int NV = (KC * OW + NT * VE - 1) / (NT * VE);  // 16-byte loads per thread, chunk
   static_assert(KC * OW % VE == 0, "tile / vector size mismatch");
   __shared__ TI cs[2][KC][LW];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
-  const long p0 = ((long)blockIdx.x * 4 + wave) * 16;
+  const long p0 = ((long)blockIdx.x * WPB + wave) * 16;
   const bool live = p0 < P;  // wave-uniform
   const int comp = blockIdx.y;
   const int o0 = blockIdx.z * OW;
@@ -101,20 +102,20 @@ orbital_gemm_kernel(const TI *__restrict__ Ct, int ldc, int n_orb, const TI *__r
 #pragma unroll
   for (int t = 0; t < MT; t++) acc[t] = acc_t{0, 0, 0, 0};
 
-  // cooperative chunk load: element e = (j * 256 + tid) * VE of the KC x OW tile
+  // cooperative chunk load: element e = (j * NT + tid) * VE of the KC x OW tile
   typedef __attribute__((ext_vector_type(4))) unsigned int u4;
   u4 stage[NV];
   auto gload = [&](int k0) {
 #pragma unroll
     for (int j = 0; j < NV; j++) {
-      const int e = (j * 256 + tid) * VE, r = e / OW, c = e % OW;
+      const int e = (j * NT + tid) * VE, r = e / OW, c = e % OW;
       if (e < KC * OW) stage[j] = *(const u4 *)(Ct + (long)(k0 + r) * ldc + o0 + c);
     }
   };
   auto lstore = [&](int buf) {
 #pragma unroll
     for (int j = 0; j < NV; j++) {
-      const int e = (j * 256 + tid) * VE, r = e / OW, c = e % OW;
+      const int e = (j * NT + tid) * VE, r = e / OW, c = e % OW;
       if (e < KC * OW) *(u4 *)&cs[buf][r][c] = stage[j];
     }
   };
@@ -179,7 +180,7 @@ orbital_gemm_kernel(const TI *__restrict__ Ct, int ldc, int n_orb, const TI *__r
   if (o.s_orb == 1 && o.per_walker == 1 && !o.walker_map) {
     // point-major output (one walker's orbital row contiguous): transpose each
     // 16x16 tile through LDS so a point's 16 orbitals go out as one segment
-    __shared__ TO tile[4][16][17];
+    __shared__ TO tile[WPB][16][17];
 #pragma unroll
     for (int t = 0; t < MT; t++) {
 #pragma unroll
@@ -225,43 +226,53 @@ orbital_gemm_kernel(const TI *__restrict__ Ct, int ldc, int n_orb, const TI *__r
   }
 }
 
-template <typename TI, typename TO>
-hipError_t launch_orbital_gemm(const TI *Ct, int ldc, int n_orb, const TI *X, int ldk, long ldp,
-                               long P, int kpad, const GemmOut &o, hipStream_t st) {
-  if (P <= 0 || n_orb <= 0) return hipSuccess;
-  const unsigned gx = (unsigned)((P + 63) / 64);
+template <typename TI, typename TO, int WPB>
+static void gemm_grid(const TI *Ct, int ldc, int n_orb, const TI *X, int ldk, long ldp, long P,
+                      int kpad, const GemmOut &o, hipStream_t st) {
+  const unsigned gx = (unsigned)((P + 16 * WPB - 1) / (16 * WPB));
+  const dim3 blk(64 * WPB);
   if (n_orb > 64) {
     dim3 grid(gx, kComp, (unsigned)((n_orb + 127) / 128));
-    hipLaunchKernelGGL((orbital_gemm_kernel<TI, TO, 8>), grid, dim3(256), 0, st, Ct, ldc, n_orb,
+    hipLaunchKernelGGL((orbital_gemm_kernel<TI, TO, 8, WPB>), grid, blk, 0, st, Ct, ldc, n_orb,
                        X, ldk, ldp, P, kpad, o);
   } else if (n_orb > 32) {
-    dim3 grid(gx, kComp, 1);
-    hipLaunchKernelGGL((orbital_gemm_kernel<TI, TO, 4>), grid, dim3(256), 0, st, Ct, ldc, n_orb,
-                       X, ldk, ldp, P, kpad, o);
+    hipLaunchKernelGGL((orbital_gemm_kernel<TI, TO, 4, WPB>), dim3(gx, kComp, 1), blk, 0, st, Ct,
+                       ldc, n_orb, X, ldk, ldp, P, kpad, o);
   } else if (n_orb > 16) {
-    dim3 grid(gx, kComp, 1);
-    hipLaunchKernelGGL((orbital_gemm_kernel<TI, TO, 2>), grid, dim3(256), 0, st, Ct, ldc, n_orb,
-                       X, ldk, ldp, P, kpad, o);
+    hipLaunchKernelGGL((orbital_gemm_kernel<TI, TO, 2, WPB>), dim3(gx, kComp, 1), blk, 0, st, Ct,
+                       ldc, n_orb, X, ldk, ldp, P, kpad, o);
   } else {
-    dim3 grid(gx, kComp, 1);
-    hipLaunchKernelGGL((orbital_gemm_kernel<TI, TO, 1>), grid, dim3(256), 0, st, Ct, ldc, n_orb,
-                       X, ldk, ldp, P, kpad, o);
+    hipLaunchKernelGGL((orbital_gemm_kernel<TI, TO, 1, WPB>), dim3(gx, kComp, 1), blk, 0, st, Ct,
+                       ldc, n_orb, X, ldk, ldp, P, kpad, o);
   }
+}
+
+// wpb: waves (16-point tiles) per workgroup, 4 or 2.  Every output element
+// accumulates the same k-chunks in the same order either way, so the choice
+// changes speed only, never a result bit.
+template <typename TI, typename TO>
+hipError_t launch_orbital_gemm(const TI *Ct, int ldc, int n_orb, const TI *X, int ldk, long ldp,
+                               long P, int kpad, const GemmOut &o, hipStream_t st, int wpb) {
+  if (P <= 0 || n_orb <= 0) return hipSuccess;
+  if (wpb == 2)
+    gemm_grid<TI, TO, 2>(Ct, ldc, n_orb, X, ldk, ldp, P, kpad, o, st);
+  else
+    gemm_grid<TI, TO, 4>(Ct, ldc, n_orb, X, ldk, ldp, P, kpad, o, st);
   return hipGetLastError();
 }
 
 template hipError_t launch_orbital_gemm<double, double>(const double *, int, int, const double *,
                                                         int, long, long, int, const GemmOut &,
-                                                        hipStream_t);
+                                                        hipStream_t, int);
 template hipError_t launch_orbital_gemm<float, float>(const float *, int, int, const float *, int,
                                                       long, long, int, const GemmOut &,
-                                                      hipStream_t);
+                                                      hipStream_t, int);
 template hipError_t launch_orbital_gemm<double, float>(const double *, int, int, const double *,
                                                        int, long, long, int, const GemmOut &,
-                                                       hipStream_t);
+                                                       hipStream_t, int);
 template hipError_t launch_orbital_gemm<__hip_bfloat16, float>(const __hip_bfloat16 *, int, int,
                                                                const __hip_bfloat16 *, int, long,
                                                                long, int, const GemmOut &,
-                                                               hipStream_t);
+                                                               hipStream_t, int);
 
 }  // namespace qmcb

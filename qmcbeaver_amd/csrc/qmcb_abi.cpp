@@ -49,6 +49,10 @@ struct qmcb_ctx {
   DevSystem dsys{};
   std::vector<void *> allocs;
   size_t dev_bytes = 0;  // device memory requested by this context
+  // single-electron orbital GEMM: waves per workgroup chosen by tune_gemm
+  // (0 = not tuned yet) and the batch size it was tuned at, per spin
+  int gemm_wpb[2] = {0, 0};
+  int gemm_tuned_num[2] = {0, 0};
   size_t ts = 8, tx = 8, to = 8;  // bytes: derivative storage, GEMM operand, GEMM output
 
   // resident walker state
@@ -295,6 +299,37 @@ MdetArgs mdet_args(qmcb_ctx *ctx, int m, int mode, const qmcb_results &out) {
   return a;
 }
 
+// ---- tile autotuning -----------------------------------------------------------
+// The single-electron orbital GEMM has one row per walker, so its grid is
+// small (num / 64 workgroups per component) and how evenly it spreads over
+// the CUs decides its speed.  The first move of a spin (and any later call
+// whose batch differs by more than 2x from the tuned one) times the 4-wave
+// and 2-wave workgroup variants on the real operands, three launches each,
+// and keeps the faster.  Both variants produce identical bits.
+template <typename F>
+qmcb_status tune_gemm(qmcb_ctx *ctx, int sp, int num, F &&gemm) {
+  hipEvent_t e0, e1;
+  QCHK(hipEventCreate(&e0));
+  QCHK(hipEventCreate(&e1));
+  const int cand[2] = {4, 2};
+  float best_ms[2] = {1e30f, 1e30f};
+  for (int c = 0; c < 2; c++)
+    for (int rep = 0; rep < 3; rep++) {
+      QCHK(hipEventRecord(e0, ctx->stream));
+      QCHK(gemm(cand[c]));
+      QCHK(hipEventRecord(e1, ctx->stream));
+      QCHK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      QCHK(hipEventElapsedTime(&ms, e0, e1));
+      best_ms[c] = std::min(best_ms[c], ms);
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  ctx->gemm_wpb[sp] = best_ms[1] < best_ms[0] ? cand[1] : cand[0];
+  ctx->gemm_tuned_num[sp] = num;
+  return QMCB_OK;
+}
+
 // ---- precision dispatch ------------------------------------------------------
 // TS: derivative storage, TX: basis/GEMM operand, TO: GEMM output
 template <typename TS, typename TX, typename TC, typename TO>
@@ -376,10 +411,17 @@ struct Engine {
     const int n = ctx->n[sp];
     GemmOut go{ctx->vrow, (long)ctx->npad[sp], 1, (long)kComp * ctx->npad[sp], 0, 1, nullptr,
                nullptr, 0};
-    TIMED("orbital_gemm", (launch_orbital_gemm<TX, TO>((const TX *)ctx->Ct[sp], ctx->npad[sp],
-                                                      ctx->nact[sp], (const TX *)ctx->X,
-                                                      ctx->kpad, ctx->ldp, (long)num, ctx->kpad,
-                                                      go, ctx->stream)));
+    auto gemm = [&](int wpb) {
+      return launch_orbital_gemm<TX, TO>((const TX *)ctx->Ct[sp], ctx->npad[sp], ctx->nact[sp],
+                                         (const TX *)ctx->X, ctx->kpad, ctx->ldp, (long)num,
+                                         ctx->kpad, go, ctx->stream, wpb);
+    };
+    if (ctx->gemm_wpb[sp] == 0 || num > 2 * ctx->gemm_tuned_num[sp] ||
+        2 * num < ctx->gemm_tuned_num[sp]) {
+      qmcb_status ts = tune_gemm(ctx, sp, num, gemm);
+      if (ts != QMCB_OK) return ts;
+    }
+    TIMED("orbital_gemm", gemm(ctx->gemm_wpb[sp]));
     (void)n;
     if (ctx->cusp) TIMED("cusp", (launch_cusp<TO>(ctx->cuspd[sp], pts, go, ctx->stream)));
     return QMCB_OK;

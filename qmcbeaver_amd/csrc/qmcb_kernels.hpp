@@ -54,7 +54,8 @@ hipError_t launch_basis(const DevSystem &sys, const PointSet &pts, TS *X, int ld
 // TI: operand type (double, float, bf16); TO: output type (double/float)
 template <typename TI, typename TO>
 hipError_t launch_orbital_gemm(const TI *Ct, int ldc, int n_orb, const TI *X, int ldk, long ldp,
-                               long P, int kpad, const GemmOut &o, hipStream_t st);
+                               long P, int kpad, const GemmOut &o, hipStream_t st,
+                               int wpb = 4);
 
 // ---- kernel (c): Slater inverse, SM updates, traces ----
 struct SlaterArgs {
