@@ -437,6 +437,16 @@ hipError_t launch_traces_all(const SlaterArgs &a, const int *walkers, int num, h
 #endif
 
 
+// threads per walker for the 32- and 64-column tiles: one and two waves
+// (more walkers resident per CU; 5 % faster on the batched many-molecule
+// workload than two and four waves)
+#ifndef QMCB_SM_NT32
+#define QMCB_SM_NT32 64
+#endif
+#ifndef QMCB_SM_NT64
+#define QMCB_SM_NT64 128
+#endif
+
 // streamed-once operands: optionally non-temporal loads
 template <typename T>
 __device__ __forceinline__ T ld_stream(const T *p) {
@@ -914,11 +924,11 @@ hipError_t launch_sm_move(const MoveArgs &a, const int *walkers, int num, hipStr
     hipLaunchKernelGGL((sm_move_kernel<TS, TO, 16, 64>), dim3(num), dim3(64), 0, st, a, walkers,
                        num);
   else if (n <= 32)
-    hipLaunchKernelGGL((sm_move_kernel<TS, TO, 32, 128>), dim3(num), dim3(128), 0, st, a,
-                       walkers, num);
+    hipLaunchKernelGGL((sm_move_kernel<TS, TO, 32, QMCB_SM_NT32>), dim3(num), dim3(QMCB_SM_NT32),
+                       0, st, a, walkers, num);
   else if (n <= 64)
-    hipLaunchKernelGGL((sm_move_kernel<TS, TO, 64, 256>), dim3(num), dim3(256), 0, st, a,
-                       walkers, num);
+    hipLaunchKernelGGL((sm_move_kernel<TS, TO, 64, QMCB_SM_NT64>), dim3(num), dim3(QMCB_SM_NT64),
+                       0, st, a, walkers, num);
   else if (n <= 128)
     hipLaunchKernelGGL((sm_move_kernel<TS, TO, 128, 256>), dim3(num), dim3(256), 0, st, a,
                        walkers, num);
