@@ -57,11 +57,17 @@ def sm_bytes_per_move(n, N, s_d, s_o, acc):
     return reads + writes
 
 
-def run_path(s, precision, W, steps, warmup, tau, block, refresh, seed):
+def run_path(s, precision, W, steps, warmup, tau, block, refresh, seed, groups=1):
+    """Timed VMC of W walkers: `steps` steps in blocks of `block`, with the
+    walkers split into `groups` overlapping groups (bit-identical results for
+    every group count), then a second pass of the same length with one group
+    and per-kernel HIP events on the library's stream (the kernel times and
+    the roofline come from that pass; the events cost ~3 % and stay out of
+    the timed throughput)."""
     from qmcbeaver_amd import synth, Context
     R0 = synth.initial_walkers(s, W, seed)
     ctx = Context(s.pack(), precision=precision, max_walkers=W, refresh_interval=refresh,
-                  seed=seed)
+                  seed=seed, walker_groups=groups)
     ids = np.arange(W)
     t0 = time.perf_counter()
     res = ctx.evaluate(ids, R0)
@@ -72,7 +78,6 @@ def run_path(s, precision, W, steps, warmup, tau, block, refresh, seed):
         ctx.vmc_steps(W, step, warmup, tau)
         step += warmup
     ctx.sync()
-    ctx.enable_timing(True)
     lat = []
     acc = 0.0
     esum = 0.0
@@ -89,12 +94,19 @@ def run_path(s, precision, W, steps, warmup, tau, block, refresh, seed):
         done += k
     ctx.sync()
     dt = time.perf_counter() - t0
+    ctx.set_walker_groups(1)
+    ctx.enable_timing(True)
+    tk = time.perf_counter()
+    ctx.vmc_steps(W, step, steps, tau)
+    ctx.sync()
+    dtk = time.perf_counter() - tk
     kt = ctx.kernel_times()
     ctx.enable_timing(False)
     ctx.close()
     return {
         "walker_moves_per_s": W * steps / dt,
         "ms_per_step": 1e3 * dt / steps,
+        "ms_per_step_one_group_timed": 1e3 * dtk / steps,
         "block_ms": lat,
         "acceptance": acc / (W * steps),
         "mean_e_local": esum / (W * steps),
@@ -233,6 +245,8 @@ def main():
     ap.add_argument("--tau", type=float, default=0.01)
     ap.add_argument("--block", type=int, default=16, help="steps per timed block")
     ap.add_argument("--refresh", type=int, default=64)
+    ap.add_argument("--groups", type=int, default=3,
+                    help="walker groups on overlapping streams (results are bit-identical)")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp32store", "bf16", "fp64"])
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -311,18 +325,19 @@ def main():
     s = synth.make_system(args.config)
     n, N = s.n_up, s.n_elec
     mixed = run_path(s, args.precision, args.walkers, args.steps, args.warmup, args.tau,
-                     args.block, args.refresh if args.precision != "fp64" else 0, 1234)
+                     args.block, args.refresh if args.precision != "fp64" else 0, 1234,
+                     args.groups)
     fp64 = None
     store = None
     bf16 = None
     if not args.no_fp64 and args.precision != "fp64":
         fp64 = run_path(s, "fp64", args.walkers, args.steps, args.warmup, args.tau, args.block,
-                        0, 1234)
+                        0, 1234, args.groups)
         if args.precision == "fp32":
             store = run_path(s, "fp32store", args.walkers, args.steps, args.warmup, args.tau,
-                             args.block, args.refresh, 1234)
+                             args.block, args.refresh, 1234, args.groups)
             bf16 = run_path(s, "bf16", args.walkers, args.steps, args.warmup, args.tau,
-                            args.block, args.refresh, 1234)
+                            args.block, args.refresh, 1234, args.groups)
 
     s_d = 8 if args.precision == "fp64" else 4
     kt = mixed["kernels"].get("sm_move", {"ms": float("nan"), "launches": 1})
@@ -351,6 +366,7 @@ def main():
                          f"FP64 re-inversion every {args.refresh} moves"
                          if args.precision != "fp64" else "FP64 throughout",
             "walkers": args.walkers, "electrons": N, "basis_functions": s.n_basis,
+            "walker_groups": args.groups,
         },
         "fp64": None if fp64 is None else {
             "value": fp64["walker_moves_per_s"], "ms_per_step": fp64["ms_per_step"],
@@ -364,6 +380,7 @@ def main():
                              "block_steps": args.block},
         "acceptance": mixed["acceptance"],
         "mean_e_local": mixed["mean_e_local"],
+        "ms_per_step_one_group_with_kernel_events": mixed["ms_per_step_one_group_timed"],
         "roofline": {"kernel": "sm_move", "bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(args.precision)

@@ -118,6 +118,12 @@ typedef struct qmcb_options {
   int64_t basis_scratch_bytes; /* cap on the basis-value scratch; 0 = 2 GiB.
                                All-electron evaluates run in chunks that fit;
                                at least one point per walker is kept          */
+  int32_t walker_groups;    /* qmcb_vmc_steps with one determinant: split the
+                               walkers into this many groups (at most 8, each
+                               of at least 256 walkers), each enqueued on a
+                               stream of its own so that their kernels overlap;
+                               results are bit-identical for every value.
+                               0 or 1 = one group                             */
 } qmcb_options;
 
 /* Caller-owned outputs, one row per evaluated walker.  Any pointer may be
@@ -260,6 +266,10 @@ qmcb_status qmcb_dmc_population(qmcb_ctx *ctx, int32_t cap, int32_t *slots,
                                 double *weights, int32_t *count);
 
 qmcb_status qmcb_sync(qmcb_ctx *ctx);
+
+/* Change qmcb_options.walker_groups of a live context (0..8); takes effect
+ * at the next qmcb_vmc_steps.  No reference counterpart (a scheduling knob). */
+qmcb_status qmcb_set_walker_groups(qmcb_ctx *ctx, int32_t groups);
 
 /* ---- device memory ------------------------------------------------------
  * qmcb_device_bytes: device memory the context holds (requested sizes).

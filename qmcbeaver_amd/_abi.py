@@ -44,7 +44,7 @@ class Options(C.Structure):
                 ("drift_mod", C.c_int32), ("refresh_interval", C.c_int32),
                 ("dt", C.c_double), ("umrigar_a", C.c_double),
                 ("seed", C.c_uint64), ("device", C.c_int32), ("replace_cusps", C.c_int32),
-                ("basis_scratch_bytes", C.c_int64)]
+                ("basis_scratch_bytes", C.c_int64), ("walker_groups", C.c_int32)]
 
 
 class Results(C.Structure):

@@ -27,7 +27,7 @@ EXPORTS = ["qmcb_abi_version", "qmcb_create", "qmcb_destroy", "qmcb_last_error",
            "qmcb_property_destroy", "qmcb_property_zero", "qmcb_property_add",
            "qmcb_property_add_series", "qmcb_property_merge", "qmcb_property_reweight",
            "qmcb_property_summary_get", "qmcb_property_level", "qmcb_cusp_fit",
-           "qmcb_device_bytes", "qmcb_plan_device_bytes"]
+           "qmcb_device_bytes", "qmcb_plan_device_bytes", "qmcb_set_walker_groups"]
 
 _lib = None
 
@@ -82,6 +82,8 @@ def lib():
     L.qmcb_dmc_population.argtypes = [vp, C.c_int32, ip, dp, C.POINTER(C.c_int32)]
     L.qmcb_cusp_fit.restype = C.c_int
     L.qmcb_cusp_fit.argtypes = [C.POINTER(SystemStruct), C.c_int32, C.POINTER(CuspOrbital)]
+    L.qmcb_set_walker_groups.restype = C.c_int
+    L.qmcb_set_walker_groups.argtypes = [vp, C.c_int32]
     L.qmcb_device_bytes.restype = C.c_int
     L.qmcb_device_bytes.argtypes = [vp, C.POINTER(C.c_int64)]
     L.qmcb_plan_device_bytes.restype = C.c_int
@@ -126,7 +128,7 @@ def cusp_fit(packed: Dict, spin: int = 0) -> np.ndarray:
 
 def _options(precision, max_walkers, drift_mod=QMCB_DRIFT_NONE, dt=0.1, umrigar_a=0.5,
              refresh_interval=0, seed=12345, device=0, replace_cusps=False,
-             basis_scratch_bytes=0) -> Options:
+             basis_scratch_bytes=0, walker_groups=0) -> Options:
     o = Options()
     o.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
     o.max_walkers = max_walkers
@@ -138,16 +140,18 @@ def _options(precision, max_walkers, drift_mod=QMCB_DRIFT_NONE, dt=0.1, umrigar_
     o.device = device
     o.replace_cusps = 1 if replace_cusps else 0
     o.basis_scratch_bytes = int(basis_scratch_bytes)
+    o.walker_groups = int(walker_groups)
     return o
 
 
 def plan_device_bytes(packed: Dict, *, precision: str = "fp64", max_walkers: int = 1,
-                      replace_cusps: bool = False, basis_scratch_bytes: int = 0) -> int:
+                      replace_cusps: bool = False, basis_scratch_bytes: int = 0,
+                      walker_groups: int = 0) -> int:
     """Device bytes a Context of this system and these options will hold
     after its first evaluate (host-only projection, no device needed)."""
     buf = SystemBuffers(packed)
     o = _options(precision, max_walkers, replace_cusps=replace_cusps,
-                 basis_scratch_bytes=basis_scratch_bytes)
+                 basis_scratch_bytes=basis_scratch_bytes, walker_groups=walker_groups)
     b = C.c_int64()
     st = lib().qmcb_plan_device_bytes(C.byref(buf.struct), C.byref(o), C.byref(b))
     if st != 0:
@@ -174,13 +178,14 @@ class Context:
     def __init__(self, packed: Dict, *, precision: str = "fp64", max_walkers: int = 1,
                  drift_mod: int = QMCB_DRIFT_NONE, dt: float = 0.1, umrigar_a: float = 0.5,
                  refresh_interval: int = 0, seed: int = 12345, device: int = 0,
-                 replace_cusps: bool = False, basis_scratch_bytes: int = 0):
+                 replace_cusps: bool = False, basis_scratch_bytes: int = 0,
+                 walker_groups: int = 0):
         self._buf = SystemBuffers(packed)
         self.N = self._buf.N
         self.n_up = int(packed["n_up"])
         self.max_walkers = max_walkers
         o = _options(precision, max_walkers, drift_mod, dt, umrigar_a, refresh_interval, seed,
-                     device, replace_cusps, basis_scratch_bytes)
+                     device, replace_cusps, basis_scratch_bytes, walker_groups)
         self._opt = o
         h = C.c_void_p()
         st = lib().qmcb_create(C.byref(self._buf.struct), C.byref(o), C.byref(h))
@@ -279,6 +284,10 @@ class Context:
 
     def sync(self):
         self._check(lib().qmcb_sync(self._h), "qmcb_sync")
+
+    def set_walker_groups(self, groups: int):
+        """Walker groups of the next vmc_steps calls (results do not change)."""
+        self._check(lib().qmcb_set_walker_groups(self._h, int(groups)), "qmcb_set_walker_groups")
 
     def device_bytes(self) -> int:
         """Device memory this context holds."""

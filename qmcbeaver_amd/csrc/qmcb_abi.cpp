@@ -42,6 +42,14 @@ struct Timing {
 struct qmcb_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // walker groups of the device VMC: one stream per group (default priority)
+  // and the identity slot map the groups index into
+  static constexpr int kMaxGroups = 8;
+  hipStream_t gstream[kMaxGroups] = {};
+  hipEvent_t gevent[kMaxGroups] = {};
+  hipEvent_t gfront[kMaxGroups] = {};  // a group's orbital rows of the current step are done
+  hipEvent_t main_event = nullptr;
+  int *d_iota = nullptr;
   qmcb_options opt{};
   int N = 0, n[2] = {0, 0}, start[2] = {0, 0}, nn = 0, B = 0, kpad = 0, nshell = 0, nprim = 0;
   int npad[2] = {16, 16};
@@ -306,8 +314,16 @@ MdetArgs mdet_args(qmcb_ctx *ctx, int m, int mode, const qmcb_results &out) {
 // whose batch differs by more than 2x from the tuned one) times the 4-wave
 // and 2-wave workgroup variants on the real operands, three launches each,
 // and keeps the faster.  Both variants produce identical bits.
+#ifndef QMCB_GEMM_TUNE
+#define QMCB_GEMM_TUNE 1
+#endif
 template <typename F>
 qmcb_status tune_gemm(qmcb_ctx *ctx, int sp, int num, F &&gemm) {
+  if (!QMCB_GEMM_TUNE) {
+    ctx->gemm_wpb[sp] = 4;
+    ctx->gemm_tuned_num[sp] = num;
+    return QMCB_OK;
+  }
   hipEvent_t e0, e1;
   QCHK(hipEventCreate(&e0));
   QCHK(hipEventCreate(&e1));
@@ -997,6 +1013,7 @@ size_t plan_device_bytes(const qmcb_system *s, const qmcb_options *o) {
   add(WD, 4);
   add(W, 4);
   if (multi) add(W * N * 3, 8);
+  if (o->walker_groups > 1 && !multi) add(W, 4);  // slot map of the walker groups
   return total;
 }
 
@@ -1055,6 +1072,15 @@ void qmcb_destroy(qmcb_ctx *ctx) {
   for (void *p : ctx->allocs) (void)hipFree(p);
   for (hipEvent_t e : ctx->tm.pool) (void)hipEventDestroy(e);
 
+  for (int g = 0; g < qmcb_ctx::kMaxGroups; g++) {
+    if (ctx->gstream[g]) {
+      (void)hipStreamSynchronize(ctx->gstream[g]);
+      (void)hipStreamDestroy(ctx->gstream[g]);
+    }
+    if (ctx->gevent[g]) (void)hipEventDestroy(ctx->gevent[g]);
+    if (ctx->gfront[g]) (void)hipEventDestroy(ctx->gfront[g]);
+  }
+  if (ctx->main_event) (void)hipEventDestroy(ctx->main_event);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -1197,6 +1223,71 @@ static void walk_propose_args(qmcb_ctx *ctx, ProposeArgs &pa) {
   pa.nuc_z = ctx->dsys.nuc_z;
 }
 
+// Walker groups.  The device VMC can split its walkers into G contiguous
+// groups and enqueue every group's step on a stream of its own, so that one
+// group's memory-bound Sherman-Morrison pass overlaps another group's
+// compute-bound Jastrow, basis and GEMM kernels.  A group touches only its
+// own slots (through the identity map d_iota + first) and its own batch rows
+// (the staging pointers below, offset by `first` rows); random numbers are
+// keyed by (seed, step, slot), so results are bit-identical for every G.
+// GroupView points ctx's stream and batch-row staging at one group while the
+// group's kernels are enqueued, and restores them when it goes out of scope.
+struct GroupView {
+  qmcb_ctx *ctx;
+  hipStream_t stream;
+  double *propR, *gauss, *unif, *pU, *pUx, *pUxx, *pVee, *pVen, *d_obs;
+  void *X, *vrow;
+  GroupView(qmcb_ctx *c, int g, long first, int sp) : ctx(c) {
+    stream = c->stream;
+    propR = c->propR; gauss = c->gauss; unif = c->unif;
+    pU = c->pU; pUx = c->pUx; pUxx = c->pUxx; pVee = c->pVee; pVen = c->pVen;
+    d_obs = c->d_obs; X = c->X; vrow = c->vrow;
+    c->stream = c->gstream[g];
+    c->propR += 3 * first; c->gauss += 3 * first; c->unif += first;
+    c->pU += first; c->pUx += first * c->N * 3; c->pUxx += first; c->pVee += first;
+    c->pVen += first; c->d_obs += 3 * first;
+    c->X = (char *)c->X + first * c->tx;                                  // point columns
+    c->vrow = (char *)c->vrow + first * (long)kComp * c->npad[sp] * c->to;  // [row][5][npad]
+  }
+  ~GroupView() {
+    ctx->stream = stream;
+    ctx->propR = propR; ctx->gauss = gauss; ctx->unif = unif;
+    ctx->pU = pU; ctx->pUx = pUx; ctx->pUxx = pUxx; ctx->pVee = pVee; ctx->pVen = pVen;
+    ctx->d_obs = d_obs; ctx->X = X; ctx->vrow = vrow;
+  }
+};
+
+static qmcb_status ensure_groups(qmcb_ctx *ctx, int G) {
+  if (!ctx->d_iota) {
+    std::vector<int> iota(ctx->Wc);
+    for (int i = 0; i < ctx->Wc; i++) iota[i] = i;
+    QCHK(upload(ctx, (const int **)&ctx->d_iota, iota.data(), iota.size()));
+  }
+  if (!ctx->main_event) QCHK(hipEventCreateWithFlags(&ctx->main_event, hipEventDisableTiming));
+  for (int g = 0; g < G; g++) {
+    if (!ctx->gstream[g]) QCHK(hipStreamCreateWithFlags(&ctx->gstream[g], hipStreamNonBlocking));
+    if (!ctx->gevent[g]) QCHK(hipEventCreateWithFlags(&ctx->gevent[g], hipEventDisableTiming));
+    if (!ctx->gfront[g]) QCHK(hipEventCreateWithFlags(&ctx->gfront[g], hipEventDisableTiming));
+  }
+  return QMCB_OK;
+}
+
+// every group stream waits for the work queued so far on the main stream
+static qmcb_status groups_wait_main(qmcb_ctx *ctx, int G) {
+  QCHK(hipEventRecord(ctx->main_event, ctx->stream));
+  for (int g = 0; g < G; g++) QCHK(hipStreamWaitEvent(ctx->gstream[g], ctx->main_event, 0));
+  return QMCB_OK;
+}
+
+// the main stream waits for every group's queued work
+static qmcb_status main_wait_groups(qmcb_ctx *ctx, int G) {
+  for (int g = 0; g < G; g++) {
+    QCHK(hipEventRecord(ctx->gevent[g], ctx->gstream[g]));
+    QCHK(hipStreamWaitEvent(ctx->stream, ctx->gevent[g], 0));
+  }
+  return QMCB_OK;
+}
+
 static qmcb_status series_reserve(qmcb_ctx *ctx, int64_t first_step, int32_t n_steps) {
   const int64_t N = ctx->N;
   const int64_t first = ((first_step + N - 1) / N) * N;  // first multiple of N >= first_step
@@ -1239,9 +1330,16 @@ qmcb_status qmcb_vmc_steps(qmcb_ctx *ctx, int32_t num, int64_t first_step, int32
     }
   }
   qmcb_results none{};
-  for (int s = 0; s < n_steps; s++) {
-    const long step = first_step + s;
-    const int m = (int)(step % ctx->N);
+  // walker groups: single determinant only; at least 256 walkers per group
+  int G = ctx->multi ? 1 : std::max(1, std::min<int>(ctx->opt.walker_groups, qmcb_ctx::kMaxGroups));
+  while (G > 1 && num / G < 256) G--;
+  if (G > 1) {
+    qmcb_status gs = ensure_groups(ctx, G);
+    if (gs != QMCB_OK) return gs;
+    gs = groups_wait_main(ctx, G);
+    if (gs != QMCB_OK) return gs;
+  }
+  auto propose = [&](long step, int m, const int *d_map, int cnt) -> qmcb_status {
     ProposeArgs pa{};
     pa.N = ctx->N;
     pa.moved = m;
@@ -1258,8 +1356,58 @@ qmcb_status qmcb_vmc_steps(qmcb_ctx *ctx, int32_t num, int64_t first_step, int32
     pa.propR = ctx->propR;
     pa.gauss = ctx->gauss;
     pa.unif = ctx->unif;
-    TIMED("propose", launch_propose(pa, nullptr, num, ctx->stream));
-    qmcb_status st = DISPATCH(ctx, move_front(ctx, nullptr, num, m));
+    TIMED("propose", launch_propose(pa, d_map, cnt, ctx->stream));
+    return QMCB_OK;
+  };
+  for (int s = 0; s < n_steps; s++) {
+    const long step = first_step + s;
+    const int m = (int)(step % ctx->N);
+    const int sp = m < ctx->n[0] ? 0 : 1;
+    // FP64 re-inversion of a spin's Slater matrices after refresh_interval
+    // Sherman-Morrison updates of that spin (all walkers move in lockstep)
+    bool do_refresh = false;
+    if (ctx->opt.refresh_interval > 0 && ++ctx->sm_updates[sp] >= ctx->opt.refresh_interval) {
+      ctx->sm_updates[sp] = 0;
+      do_refresh = true;
+    }
+    qmcb_status st = QMCB_OK;
+    if (G > 1) {
+      for (int g = 0; g < G; g++) {
+        const long first = (long)num * g / G, cnt = (long)num * (g + 1) / G - first;
+        const int *d_map = ctx->d_iota + first;
+        GroupView view(ctx, g, first, sp);
+        // stagger: group g's front half (proposal, Jastrow, basis, GEMM) of a
+        // step starts after group g-1's front half of the same step, so each
+        // group trails the one before it and its front overlaps that group's
+        // Sherman-Morrison pass; group 0 runs free
+        if (g > 0) QCHK(hipStreamWaitEvent(ctx->stream, ctx->gfront[g - 1], 0));
+        st = propose(step, m, d_map, (int)cnt);
+        if (st == QMCB_OK) st = DISPATCH(ctx, move_front(ctx, d_map, (int)cnt, m));
+        if (st == QMCB_OK) QCHK(hipEventRecord(ctx->gfront[g], ctx->stream));
+        if (st == QMCB_OK) {
+          MoveArgs a = DISPATCH(ctx, move_args(ctx, m, 1, none));
+          a.tau = tau;
+          if (m == 0) a.obs = ctx->d_obs;
+          st = DISPATCH(ctx, sm(ctx, a, d_map, (int)cnt));
+        }
+        if (st == QMCB_OK && do_refresh) st = DISPATCH(ctx, refresh(ctx, d_map, (int)cnt, 1 << sp));
+        if (st != QMCB_OK) return st;
+      }
+      if (m == 0) {
+        // the sweep's observable row sums every walker in one fixed order
+        st = main_wait_groups(ctx, G);
+        if (st != QMCB_OK) return st;
+        TIMED("series", launch_series_reduce(ctx->d_obs, num,
+                                             ctx->d_series + 4L * ctx->series_len++,
+                                             ctx->stream));
+        st = groups_wait_main(ctx, G);
+        if (st != QMCB_OK) return st;
+      }
+      continue;
+    }
+    st = propose(step, m, nullptr, num);
+    if (st != QMCB_OK) return st;
+    st = DISPATCH(ctx, move_front(ctx, nullptr, num, m));
     if (st != QMCB_OK) return st;
     if (ctx->multi) {
       // several determinants: stage every determinant, combine and accept on
@@ -1278,14 +1426,14 @@ qmcb_status qmcb_vmc_steps(qmcb_ctx *ctx, int32_t num, int64_t first_step, int32
     if (m == 0)
       TIMED("series", launch_series_reduce(ctx->d_obs, num, ctx->d_series + 4L * ctx->series_len++,
                                            ctx->stream));
-    // FP64 re-inversion of a spin's Slater matrices after refresh_interval
-    // Sherman-Morrison updates of that spin (all walkers move in lockstep)
-    const int sp = m < ctx->n[0] ? 0 : 1;
-    if (ctx->opt.refresh_interval > 0 && ++ctx->sm_updates[sp] >= ctx->opt.refresh_interval) {
-      ctx->sm_updates[sp] = 0;
+    if (do_refresh) {
       st = DISPATCH(ctx, refresh(ctx, nullptr, num, 1 << sp));
       if (st != QMCB_OK) return st;
     }
+  }
+  if (G > 1) {
+    qmcb_status gs = main_wait_groups(ctx, G);
+    if (gs != QMCB_OK) return gs;
   }
   if (stats) {
     std::vector<double> a(num), b(num), c(num), d(num);
@@ -1587,6 +1735,12 @@ qmcb_status qmcb_sync(qmcb_ctx *ctx) {
   QCHK(hipSetDevice(ctx->device));
   QCHK(hipStreamSynchronize(ctx->stream));
   tm_collect(ctx);
+  return QMCB_OK;
+}
+
+qmcb_status qmcb_set_walker_groups(qmcb_ctx *ctx, int32_t groups) {
+  if (!ctx || groups < 0 || groups > qmcb_ctx::kMaxGroups) return QMCB_INVALID_ARG;
+  ctx->opt.walker_groups = groups;
   return QMCB_OK;
 }
 

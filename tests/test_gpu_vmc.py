@@ -85,3 +85,33 @@ def test_vmc_deterministic_and_seeded():
             stats.append((st.sum_e_local, st.sum_accept))
     assert stats[0] == stats[1]
     assert stats[0] != stats[2]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_vmc_walker_groups_bit_identical(precision):
+    """Walker groups on their own streams (overlapping kernels) change the
+    schedule, not the arithmetic: positions, energy and acceptance sums and
+    the per-sweep series equal the one-group run bit for bit, across sweep
+    boundaries and FP64 re-inversions."""
+    s = synth.make_system(1)
+    W, N = 1024, s.n_elec
+    R0 = synth.initial_walkers(s, W, 23)
+    out = []
+    for groups in (1, 4, 3):
+        say("vmc walker groups", precision, groups)
+        with Context(s.pack(), precision=precision, max_walkers=W, seed=5,
+                     refresh_interval=16 if precision != "fp64" else 0,
+                     walker_groups=groups) as ctx:
+            ctx.evaluate(np.arange(W), R0)
+            st = ctx.vmc_steps(W, 0, N + 9, 0.02)
+            rows = ctx.sample_series()
+            st2 = ctx.vmc_steps(W, N + 9, N, 0.02)
+            rows2 = ctx.sample_series()
+            out.append((ctx.positions(W), (st.sum_e_local, st.sum_accept, st.sum_prop_e_local,
+                                           st2.sum_e_local, st2.sum_accept), rows, rows2))
+    for o in out[1:]:
+        np.testing.assert_array_equal(o[0], out[0][0])
+        assert o[1] == out[0][1]
+        np.testing.assert_array_equal(o[2], out[0][2])
+        np.testing.assert_array_equal(o[3], out[0][3])
+
