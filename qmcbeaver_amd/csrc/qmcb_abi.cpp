@@ -1552,6 +1552,15 @@ qmcb_status qmcb_dmc_steps(qmcb_ctx *ctx, int64_t first_step, int32_t n_steps,
   std::vector<double> wts(W), acc_e(W), acc_w(W);
   const CopyArgs cargs = walker_copy_args(ctx);
   long moves = 0;
+  // walker groups (single determinant): as in qmcb_vmc_steps, over the active list
+  const int G =
+      ctx->multi ? 1 : std::max(1, std::min<int>(ctx->opt.walker_groups, qmcb_ctx::kMaxGroups));
+  if (G > 1) {
+    qmcb_status gs = ensure_groups(ctx, G);
+    if (gs != QMCB_OK) return gs;
+    gs = groups_wait_main(ctx, G);
+    if (gs != QMCB_OK) return gs;
+  }
   for (int s = 0; s < n_steps; s++) {
     const long step = first_step + s;
     const int m = (int)(step % ctx->N);
@@ -1560,44 +1569,70 @@ qmcb_status qmcb_dmc_steps(qmcb_ctx *ctx, int64_t first_step, int32_t n_steps,
       ctx->err = "DMC population died out";
       return QMCB_BAD_STATE;
     }
-    ProposeArgs pa{};
-    pa.N = ctx->N;
-    pa.moved = m;
-    pa.n_up = ctx->n[0];
-    pa.R = ctx->R;
-    pa.gradPR[0] = ctx->gradPR[0];
-    pa.gradPR[1] = ctx->gradPR[1];
-    pa.Ux = ctx->Ux;
-    pa.cgrad = ctx->multi ? ctx->cgrad : nullptr;
-    pa.tau = prm->tau;
-    walk_propose_args(ctx, pa);
-    pa.seed = ctx->opt.seed;
-    pa.step = step;
-    pa.propR = ctx->propR;
-    pa.gauss = ctx->gauss;
-    pa.unif = ctx->unif;
-    TIMED("propose", launch_propose(pa, ctx->d_active, pop, ctx->stream));
-    qmcb_status sst = DISPATCH(ctx, move_front(ctx, ctx->d_active, pop, m));
-    if (sst != QMCB_OK) return sst;
-    qmcb_results none{};
-    MoveArgs a = DISPATCH(ctx, move_args(ctx, m, 2, none));
-    a.tau = prm->tau;
-    a.e_trial = ctx->e_trial;
-    a.e_est = ctx->e_est;
-    a.energy_mod = prm->energy_modification;
-    a.tau_eff = prm->tau;
-    a.reweight = (m == 0 && step >= prm->reweight_after) ? 1 : 0;
-    if (m == 0) a.obs = ctx->d_obs;
-    sst = DISPATCH(ctx, sm(ctx, a, ctx->d_active, pop));
-    if (sst != QMCB_OK) return sst;
+    const int sp = m < ctx->n[0] ? 0 : 1;
+    bool do_refresh = false;
+    if (ctx->opt.refresh_interval > 0 && ++ctx->sm_updates[sp] >= ctx->opt.refresh_interval) {
+      ctx->sm_updates[sp] = 0;
+      do_refresh = true;
+    }
+    // one group's step over cnt rows of the active list (g < 0: no groups)
+    auto group_step = [&](const int *d_map, int cnt, int g) -> qmcb_status {
+      ProposeArgs pa{};
+      pa.N = ctx->N;
+      pa.moved = m;
+      pa.n_up = ctx->n[0];
+      pa.R = ctx->R;
+      pa.gradPR[0] = ctx->gradPR[0];
+      pa.gradPR[1] = ctx->gradPR[1];
+      pa.Ux = ctx->Ux;
+      pa.cgrad = ctx->multi ? ctx->cgrad : nullptr;
+      pa.tau = prm->tau;
+      walk_propose_args(ctx, pa);
+      pa.seed = ctx->opt.seed;
+      pa.step = step;
+      pa.propR = ctx->propR;
+      pa.gauss = ctx->gauss;
+      pa.unif = ctx->unif;
+      TIMED("propose", launch_propose(pa, d_map, cnt, ctx->stream));
+      qmcb_status gst = DISPATCH(ctx, move_front(ctx, d_map, cnt, m));
+      if (gst != QMCB_OK) return gst;
+      if (g >= 0) QCHK(hipEventRecord(ctx->gfront[g], ctx->stream));
+      qmcb_results none{};
+      MoveArgs a = DISPATCH(ctx, move_args(ctx, m, 2, none));
+      a.tau = prm->tau;
+      a.e_trial = ctx->e_trial;
+      a.e_est = ctx->e_est;
+      a.energy_mod = prm->energy_modification;
+      a.tau_eff = prm->tau;
+      a.reweight = (m == 0 && step >= prm->reweight_after) ? 1 : 0;
+      if (m == 0) a.obs = ctx->d_obs;
+      gst = DISPATCH(ctx, sm(ctx, a, d_map, cnt));
+      if (gst != QMCB_OK) return gst;
+      if (do_refresh) gst = DISPATCH(ctx, refresh(ctx, d_map, cnt, 1 << sp));
+      return gst;
+    };
+    qmcb_status sst = QMCB_OK;
+    if (G > 1 && pop / G >= 256) {
+      for (int g = 0; g < G; g++) {
+        const long first = (long)pop * g / G, cnt = (long)pop * (g + 1) / G - first;
+        GroupView view(ctx, g, first, sp);
+        if (g > 0) QCHK(hipStreamWaitEvent(ctx->stream, ctx->gfront[g - 1], 0));
+        sst = group_step(ctx->d_active + first, (int)cnt, g);
+        if (sst != QMCB_OK) return sst;
+      }
+      sst = main_wait_groups(ctx, G);
+      if (sst != QMCB_OK) return sst;
+    } else {
+      sst = group_step(ctx->d_active, pop, -1);
+      if (sst != QMCB_OK) return sst;
+    }
     if (m == 0)
       TIMED("series", launch_series_reduce(ctx->d_obs, pop, ctx->d_series + 4L * ctx->series_len++,
                                            ctx->stream));
     moves += pop;
-    const int sp = m < ctx->n[0] ? 0 : 1;
-    if (ctx->opt.refresh_interval > 0 && ++ctx->sm_updates[sp] >= ctx->opt.refresh_interval) {
-      ctx->sm_updates[sp] = 0;
-      sst = DISPATCH(ctx, refresh(ctx, ctx->d_active, pop, 1 << sp));
+    if (G > 1) {
+      // the groups of the next step start after everything queued so far
+      sst = groups_wait_main(ctx, G);
       if (sst != QMCB_OK) return sst;
     }
     if (m != 0) continue;

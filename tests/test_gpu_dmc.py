@@ -97,3 +97,31 @@ def test_dmc_h2_reaches_exact_ground_state(modified):
         assert s.decorr_depth >= 0 and s.std_dev < 2e-3
         assert abs(s.average - (-1.174476)) < max(4 * s.std_dev, 1.5e-3)
         assert e_vmc > s.average + 0.005
+
+
+def test_dmc_walker_groups_bit_identical():
+    """Walker groups in the device DMC (overlapping streams over the active
+    list) give the same populations, weights, trial energies and positions
+    bit for bit as one group, through splits, fusions and re-inversions
+    (H2 6-31G**: stable populations; aggressive thresholds force branching)."""
+    packed, _, _ = load("h2_631gss")
+    W0, cap = 900, 2700
+    R0 = configs(packed, W0, 45)
+    out = []
+    for groups in (1, 3):
+        print("case dmc walker groups", groups, flush=True)
+        with Context(packed, precision="fp64", max_walkers=cap, seed=9, refresh_interval=8,
+                     walker_groups=groups) as ctx:
+            res = ctx.evaluate(np.arange(W0), R0)
+            ctx.dmc_init(W0)
+            e0 = float(np.mean(res["e_local"]))
+            st = ctx.dmc_steps(0, 61, 0.01, e0, target=W0, branch_threshold=1.05,
+                               fusion_threshold=0.95, reweight_after=0)
+            slots, w = ctx.dmc_population()
+            out.append(((st.sum_w_e_local, st.sum_weights, st.population, st.n_split, st.n_fused,
+                         st.n_killed, st.e_trial), slots, w, ctx.positions(cap)[slots],
+                        ctx.sample_series()))
+    assert out[0][0][3] > 0 and out[0][0][4] > 0
+    assert out[1][0] == out[0][0]
+    for k in (1, 2, 3, 4):
+        np.testing.assert_array_equal(out[1][k], out[0][k])
