@@ -204,7 +204,7 @@ qmcb_status qmcb_get_positions(qmcb_ctx *ctx, int32_t num, double *R);
  * electron (step mod N) of every walker in slots [0, num), evaluates the
  * full contract at the proposal, and accepts or rejects on the device with a
  * counter-based RNG keyed by (seed, slot, step).  The drift is gradPsiRatio
- * (qmcb_options.drift_mod QMCB_DRIFT_NONE) or, with one determinant, its
+ * (qmcb_options.drift_mod QMCB_DRIFT_NONE) or its
  * umrigar93_unequalelectrons modification (QMCB_DRIFT_UMRIGAR93_UNEQUAL, the
  * reference's default QF_modification_type; QMCSCFJastrow.cpp:368-443) with
  * the step's tau as its time step, in the proposal and the Green's function
@@ -227,8 +227,9 @@ qmcb_status qmcb_vmc_steps(qmcb_ctx *ctx, int32_t num, int64_t first_step,
  * (QMCWalker.cpp:1060-1185), non-unit-weight branching after each sweep
  * (QMCRun.cpp:740-814, 1179-1206) and the trial-energy feedback
  * E_T = E_est - pop_control * ln(W_total / target)
- * (QMCManager.cpp:2263-2310).  Walker states are copied on the device when a
- * walker splits; the population lives in the context's slots. */
+ * (QMCManager.cpp:2263-2310).  Walker states (with several determinants,
+ * every determinant's record) are copied on the device when a walker splits;
+ * the population lives in the context's slots. */
 typedef struct qmcb_dmc_params {
   double tau;               /* time step                                       */
   double e_trial;           /* initial trial energy (used when first_step == 0

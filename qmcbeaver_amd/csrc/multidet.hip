@@ -222,7 +222,9 @@ mdet_assemble_kernel(MdetArgs A, const int *walkers, int num) {
   const double *Uxc = A.Ux + w0 * N * 3;
   const double *Rw = A.R + w0 * N * 3;
   const double tau = A.tau;
+  const bool qf = staged && A.mode >= 1 && A.drift_mod == QMCB_DRIFT_UMRIGAR93_UNEQUAL;
   double lsum = 0.0, gsq = 0.0, fwd = 0.0, rev = 0.0, dummy = 0.0;
+  double vbt = 0.0, vbo = 0.0, vo = 0.0;  // |modified drift|^2 (trial, current), |v|^2 current
   for (int e = t; e < N; e += NT) {
     const int sp = e < nu ? 0 : 1;
     const int le = e - (sp == 0 ? 0 : nu);
@@ -233,7 +235,7 @@ mdet_assemble_kernel(MdetArgs A, const int *walkers, int num) {
     for (int d = 0; d < D; d++) {
       const double *g = vn.grad + ((vn.base + d) * ne + le) * 3;
       gd[0] += t_new[d] * g[0]; gd[1] += t_new[d] * g[1]; gd[2] += t_new[d] * g[2];
-      if (staged && A.mode == 1) {
+      if (staged && A.mode >= 1) {
         const double *h = vc.grad + ((vc.base + d) * ne + le) * 3;
         gc[0] += t_cur[d] * h[0]; gc[1] += t_cur[d] * h[1]; gc[2] += t_cur[d] * h[2];
       }
@@ -251,18 +253,37 @@ mdet_assemble_kernel(MdetArgs A, const int *walkers, int num) {
       double *o = A.out.grad_lnpsi + ((long)b * N + e) * 3;
       o[0] = g[0]; o[1] = g[1]; o[2] = g[2];
     }
-    if (staged && A.mode == 1) {
-      // Green's function terms (QMCWalker.cpp:343-398, 756-780), drift =
-      // gradPsiRatio (QF_modification_type none)
+    if (staged && A.mode >= 1) {
+      // Green's function terms (QMCWalker.cpp:343-398, 756-780) with the
+      // drift gradPsiRatio or its umrigar93_unequalelectrons modification
+      // (QMCSCFJastrow.cpp:368-443) at the trial (ft) and current (fo) states
+      double c[3];
+      for (int k = 0; k < 3; k++) c[k] = gc[k] + Uxc[3 * e + k];
+      double ft = 1.0, fo = 1.0;
+      if (qf) {
+        const double xo[3] = {Rw[3 * e], Rw[3 * e + 1], Rw[3 * e + 2]};
+        const int co = nearest_nucleus(xo, A.n_nuc, A.nuc_xyz);
+        fo = drift_factor_near(c, xo, co, A.nuc_xyz, A.nuc_z, tau);
+        if (e != A.moved) {
+          ft = drift_factor_near(g, xo, co, A.nuc_xyz, A.nuc_z, tau);
+        } else {
+          const double xt[3] = {A.propR[3L * b], A.propR[3L * b + 1], A.propR[3L * b + 2]};
+          ft = drift_factor_unequal(g, xt, A.n_nuc, A.nuc_xyz, A.nuc_z, tau);
+        }
+        double g2 = 0.0, c2 = 0.0;
+        for (int k = 0; k < 3; k++) { g2 += g[k] * g[k]; c2 += c[k] * c[k]; }
+        vbt += ft * ft * g2;
+        vbo += fo * fo * c2;
+        vo += c2;
+      }
       if (e != A.moved) {
         for (int k = 0; k < 3; k++) {
-          const double c = gc[k] + Uxc[3 * e + k];
-          fwd += tau * c * tau * c;
-          rev += tau * g[k] * tau * g[k];
+          fwd += tau * fo * c[k] * tau * fo * c[k];
+          rev += tau * ft * g[k] * tau * ft * g[k];
         }
       } else {
         for (int k = 0; k < 3; k++) {
-          const double tt = Rw[3 * e + k] - A.propR[3L * b + k] - tau * g[k];
+          const double tt = Rw[3 * e + k] - A.propR[3L * b + k] - tau * ft * g[k];
           rev += tt * tt;
           fwd += A.gauss[3L * b + k] * A.gauss[3L * b + k];
         }
@@ -271,6 +292,16 @@ mdet_assemble_kernel(MdetArgs A, const int *walkers, int num) {
   }
   double vals[5] = {lsum, gsq, fwd, rev, dummy};
   block_sum5<NT>(vals, scratch, s_red);
+  // |modified drift| / |drift| of the trial and current states (umrigar93
+  // energy modification of the DMC weights)
+  double mr_t = 1.0, mr_o = 1.0;
+  if (qf) {
+    __shared__ double s_red2[5];
+    double v2[5] = {vbt, vbo, vo, 0.0, 0.0};
+    block_sum5<NT>(v2, scratch, s_red2);
+    if (s_red[1] > 0.0) mr_t = sqrt(s_red2[0]) / sqrt(s_red[1]);
+    if (s_red2[2] > 0.0) mr_o = sqrt(s_red2[1]) / sqrt(s_red2[2]);
+  }
   bool sing = s_sc[2] != 0.0;
   const double gsum = s_red[1];
   if (!(gsum > 0.0) || !isfinite(gsum)) sing = true;
@@ -327,26 +358,48 @@ mdet_assemble_kernel(MdetArgs A, const int *walkers, int num) {
     if (A.cur_el) A.cur_el[w0] = EL;
     return;
   }
-  if (A.mode != 1) return;
-  // ---- Metropolis with the Green's function ratio (QMCWalker.cpp:1578-1685)
+  if (A.mode < 1) return;
+  // ---- Metropolis with the Green's function ratio (QMCWalker.cpp:1578-1685);
+  // DMC also rejects a change of the sign of the determinant sum (fixed
+  // node, :1641-1646) and reweights once per sweep as the one-determinant
+  // path does (QMCWalker.cpp:1060-1185)
   const double logpsi_old = s_sc[3] + A.U[w0];
   double p = (sing || s_sc[5] != 0.0)
                  ? 0.0
                  : exp(2.0 * (logpsi - logpsi_old) - (s_red[3] - s_red[2]) / (2.0 * tau));
   if (!(p == p)) p = 0.0;
+  if (A.mode == 2 && s_sc[1] * s_sc[4] < 0.0) p = 0.0;
   if (p > 1.0) p = 1.0;
   const bool accept = p > A.unif[b];
   const double el_old = A.cur_el[w0];
   A.mask[b] = accept ? 1 : 0;
   if (accept) A.cur_el[w0] = EL;
   const double el = accept ? EL : el_old;
-  A.acc_el[w0] += el;
-  A.acc_el2[w0] += el * el;
+  double wt = 1.0;
+  if (A.mode == 2) {
+    wt = A.weight[w0];
+    if (A.reweight) {
+      double so_, st_;
+      if (A.energy_mod) {  // umrigar93 (QMCWalker.cpp:1121-1132)
+        so_ = (A.e_trial - A.e_est) + (A.e_est - el_old) * mr_o;
+        st_ = (A.e_trial - A.e_est) + (A.e_est - EL) * mr_t;
+      } else {
+        so_ = A.e_trial - el_old;
+        st_ = A.e_trial - EL;
+      }
+      if (!(p > 0.0)) st_ = 0.0;
+      wt *= exp(A.tau_eff * (p * 0.5 * (so_ + st_) + (1.0 - p) * so_));
+      A.weight[w0] = wt;
+    }
+    A.acc_w[w0] += wt;
+  }
+  A.acc_el[w0] += wt * el;
+  A.acc_el2[w0] += wt * el * el;
   A.acc_n[w0] += accept ? 1.0 : 0.0;
   A.acc_prop_el[w0] += sing ? 0.0 : EL;
   if (A.obs) {
     A.obs[3L * b] = p > 0.0 ? p * EL + (1.0 - p) * el_old : el_old;
-    A.obs[3L * b + 1] = 1.0;
+    A.obs[3L * b + 1] = wt;
     A.obs[3L * b + 2] = p;
   }
 }

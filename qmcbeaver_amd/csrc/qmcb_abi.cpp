@@ -304,6 +304,8 @@ MdetArgs mdet_args(qmcb_ctx *ctx, int m, int mode, const qmcb_results &out) {
   a.acc_prop_el = ctx->acc_prop_el; a.cur_el = ctx->cur_el;
   a.cgrad = ctx->cgrad;
   a.st_cgrad = ctx->st_cgrad;
+  a.weight = ctx->weight;
+  a.acc_w = ctx->acc_w;
   return a;
 }
 
@@ -495,6 +497,14 @@ struct Engine {
     MdetArgs ma = mdet_args(ctx, m, mode, out);
     ma.tau = tau;
     ma.obs = obs;
+    TIMED("mdet_move", (launch_mdet_move<TS, TO>(ma, d_map, num, ctx->stream)));
+    TIMED("mdet_assemble", launch_mdet_assemble(ma, d_map, num, ctx->stream));
+    return QMCB_OK;
+  }
+
+  // the same with arguments prepared by the caller (device DMC)
+  static qmcb_status mdet_stage_args(qmcb_ctx *ctx, const MdetArgs &ma, const int *d_map,
+                                     int num) {
     TIMED("mdet_move", (launch_mdet_move<TS, TO>(ma, d_map, num, ctx->stream)));
     TIMED("mdet_assemble", launch_mdet_assemble(ma, d_map, num, ctx->stream));
     return QMCB_OK;
@@ -1205,14 +1215,12 @@ qmcb_status qmcb_get_positions(qmcb_ctx *ctx, int32_t num, double *R) {
 
 // Rows of the observable series a step range records: one per step that
 // moves electron 0.  Grows the device buffer before the step loop.
-// Drift of the device walk: gradPsiRatio (QF_modification_type none) or,
-// with one determinant, its umrigar93_unequalelectrons modification.
+// Drift of the device walk: gradPsiRatio (QF_modification_type none) or its
+// umrigar93_unequalelectrons modification.
 static bool walk_drift_ok(qmcb_ctx *ctx) {
   const int dm = ctx->opt.drift_mod;
-  if (dm == QMCB_DRIFT_NONE) return true;
-  if (dm == QMCB_DRIFT_UMRIGAR93_UNEQUAL && !ctx->multi) return true;
-  ctx->err = "device VMC/DMC drift: QF_modification_type none, or umrigar93_unequalelectrons "
-             "with one determinant";
+  if (dm == QMCB_DRIFT_NONE || dm == QMCB_DRIFT_UMRIGAR93_UNEQUAL) return true;
+  ctx->err = "device VMC/DMC drift: QF_modification_type none or umrigar93_unequalelectrons";
   return false;
 }
 
@@ -1463,23 +1471,24 @@ static CopyArgs walker_copy_args(qmcb_ctx *ctx) {
     c.bytes[c.narr] = bytes;
     c.narr++;
   };
-  const long N = ctx->N;
+  const long N = ctx->N, D = ctx->ndet;  // D per-determinant records per walker
   add(ctx->R, N * 3 * 8);
   for (int sp = 0; sp < 2; sp++) {
-    const long n = ctx->n[sp];
+    const long n = ctx->n[sp], na = ctx->nact[sp];
     if (n == 0) continue;
-    add(ctx->inv[sp], n * n * 8);
-    add(ctx->dval[sp], n * n * 8);
-    add(ctx->dmat[sp], kDer * n * n * (long)ctx->ts);
+    add(ctx->inv[sp], D * n * n * 8);
+    add(ctx->dval[sp], na * n * 8);
+    add(ctx->dmat[sp], kDer * na * n * (long)ctx->ts);
     add(ctx->ubuf[sp], kDelay * n * 8);
     add(ctx->zbuf[sp], kDelay * n * 8);
-    add(ctx->dcount[sp], 4);
-    add(ctx->logdet[sp], 8);
-    add(ctx->sgn[sp], 8);
-    add(ctx->gradPR[sp], n * 3 * 8);
-    add(ctx->lapPR[sp], 8);
-    add(ctx->singular[sp], 4);
+    add(ctx->dcount[sp], D * 4);
+    add(ctx->logdet[sp], D * 8);
+    add(ctx->sgn[sp], D * 8);
+    add(ctx->gradPR[sp], D * n * 3 * 8);
+    add(ctx->lapPR[sp], D * 8);
+    add(ctx->singular[sp], D * 4);
   }
+  if (ctx->multi) add(ctx->cgrad, N * 3 * 8);
   add(ctx->U, 8);
   add(ctx->Ux, N * 3 * 8);
   add(ctx->Uxx, 8);
@@ -1492,10 +1501,6 @@ static CopyArgs walker_copy_args(qmcb_ctx *ctx) {
 
 qmcb_status qmcb_dmc_init(qmcb_ctx *ctx, int32_t num) {
   if (!ctx || num < 1 || num > ctx->Wc) return QMCB_INVALID_ARG;
-  if (ctx->multi) {
-    ctx->err = "device DMC supports one determinant";
-    return QMCB_INVALID_ARG;
-  }
   for (int i = 0; i < num; i++)
     if (!ctx->initialized[i]) {
       ctx->err = "dmc slots must be initialised by an all-electron evaluate";
@@ -1544,6 +1549,10 @@ qmcb_status qmcb_dmc_steps(qmcb_ctx *ctx, int64_t first_step, int32_t n_steps,
   {
     qmcb_status rs = series_reserve(ctx, first_step, n_steps);
     if (rs != QMCB_OK) return rs;
+    if (ctx->multi) {
+      rs = ensure_staging(ctx);
+      if (rs != QMCB_OK) return rs;
+    }
   }
   const int target = prm->target_walkers > 0 ? prm->target_walkers : (int)ctx->active.size();
   std::mt19937_64 rng(ctx->opt.seed ^ (0x9E3779B97F4A7C15ULL * (uint64_t)(first_step + 1)));
@@ -1598,6 +1607,24 @@ qmcb_status qmcb_dmc_steps(qmcb_ctx *ctx, int64_t first_step, int32_t n_steps,
       if (gst != QMCB_OK) return gst;
       if (g >= 0) QCHK(hipEventRecord(ctx->gfront[g], ctx->stream));
       qmcb_results none{};
+      if (ctx->multi) {
+        // several determinants: stage every determinant, combine, accept
+        // and reweight on the device, commit the accepted rows
+        MdetArgs ma = mdet_args(ctx, m, 2, none);
+        ma.tau = prm->tau;
+        ma.e_trial = ctx->e_trial;
+        ma.e_est = ctx->e_est;
+        ma.energy_mod = prm->energy_modification;
+        ma.tau_eff = prm->tau;
+        ma.reweight = (m == 0 && step >= prm->reweight_after) ? 1 : 0;
+        ma.obs = m == 0 ? ctx->d_obs : nullptr;
+        gst = DISPATCH(ctx, mdet_stage_args(ctx, ma, d_map, cnt));
+        if (gst != QMCB_OK) return gst;
+        gst = DISPATCH(ctx, mdet_commit(ctx, d_map, ctx->d_mask, cnt, m));
+        if (gst != QMCB_OK) return gst;
+        if (do_refresh) gst = DISPATCH(ctx, refresh(ctx, d_map, cnt, 1 << sp));
+        return gst;
+      }
       MoveArgs a = DISPATCH(ctx, move_args(ctx, m, 2, none));
       a.tau = prm->tau;
       a.e_trial = ctx->e_trial;

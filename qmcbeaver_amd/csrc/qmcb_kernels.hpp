@@ -162,7 +162,7 @@ struct MdetArgs {
   int drift_mod;
   double dt, umrigar_a;
   qmcb_results out;      // device outputs per batch row
-  int mode;              // 0: evaluate / stage, 1: VMC Metropolis on the device
+  int mode;              // 0: evaluate / stage, 1: VMC, 2: DMC Metropolis on the device
   double tau;
   const double *gauss, *unif;
   uint8_t *mask;         // [num] accept decisions (VMC)
@@ -170,6 +170,12 @@ struct MdetArgs {
   double *obs;           // per-sweep observable rows (VMC, electron 0 moves)
   double *cgrad;         // [Wc][N][3] grad ln D of the determinant sum (resident)
   double *st_cgrad;      // [num][N][3] the same at the proposal
+  // diffusion Monte Carlo (mode 2): fixed-node rejection, weights
+  double *weight, *acc_w;
+  double e_trial, tau_eff;
+  int reweight;
+  int energy_mod;        // 0: energy_modification_type none, 1: umrigar93
+  double e_est;
 };
 template <typename TS, typename TO>
 hipError_t launch_mdet_move(const MdetArgs &a, const int *walkers, int num, hipStream_t st);

@@ -20,6 +20,7 @@ import numpy as np
 import pytest
 
 from qmcbeaver_amd import Context, ckmf
+from qmcbeaver_amd._abi import QMCB_DRIFT_UMRIGAR93_UNEQUAL
 from qmcbeaver_amd.stats import Property, series_energy
 from tests.inputs_util import GOLDEN, configs
 
@@ -127,3 +128,77 @@ def test_multidet_vmc_energy_matches_reference():
           ref["ddda_std"], flush=True)
     assert s.decorr_depth >= 0
     assert abs(s.average - ref["ddda_energy"]) < 4 * np.hypot(s.std_dev, ref["ddda_std"])
+
+
+def test_multidet_dmc_branching_keeps_states_consistent():
+    """Device DMC with several determinants: after sweeps with splits and
+    fusions (whole walker states, every determinant's record, copied on the
+    device), each live walker's resident state still describes its positions
+    (zero move against the resident state equals a fresh evaluate)."""
+    pk = load("be_mcscf")
+    W0, cap = 64, 256
+    R0 = configs(pk, W0, 62)
+    N = R0.shape[1]
+    print("case multidet DMC consistency (Be)", flush=True)
+    with Context(pk, precision="fp64", max_walkers=cap, seed=8, replace_cusps=True) as ctx, \
+            Context(pk, precision="fp64", max_walkers=cap, replace_cusps=True) as fresh:
+        res = ctx.evaluate(np.arange(W0), R0)
+        ctx.dmc_init(W0)
+        e0 = float(np.mean(res["e_local"]))
+        st = ctx.dmc_steps(0, 40 * N + 1, 0.01, e0, target=W0, branch_threshold=1.05,
+                           fusion_threshold=0.95, reweight_after=0)
+        print("pop", st.population, "split", st.n_split, "fused", st.n_fused, flush=True)
+        assert st.n_split > 0 and st.n_fused > 0
+        slots, w = ctx.dmc_population()
+        assert len(slots) == st.population and (w > 0).all()
+        R = ctx.positions(cap)[slots]
+        ref = fresh.evaluate(slots, R)
+        for e in range(N):
+            got = ctx.evaluate(slots, R, whichE=e)
+            assert rel(got["e_local"], ref["e_local"]) < 1e-8, e
+            assert rel(got["log_psi"], ref["log_psi"]) < 1e-8, e
+            ctx.accept(slots, np.zeros(len(slots), dtype=np.uint8))
+
+
+def test_multidet_dmc_energy_below_vmc():
+    """Fixed-node DMC of Be from the reference's 10-determinant MCSCF
+    orbitals and CI coefficients, with the reference's default drift
+    (umrigar93_unequalelectrons) and energy modification (umrigar93): the
+    DDDA energy lies more than 0.1 Ha below the trial function's VMC energy
+    and within 20 mHa of the exact nonrelativistic -14.66736 Ha (fixed-node
+    and tau = 0.003 time-step errors are a few mHa).  The cusp comes from the
+    orbitals' cusp replacement; the input's electron-nucleus Jastrow (which
+    carries a second -Z cusp inside 0.01 bohr) is switched off, since with
+    both the local energy has a -Z/r tail that no finite time step tames."""
+    pk = dict(load("be_mcscf"))
+    for k in ("en_up", "en_dn"):
+        a = pk[k].copy()
+        a[:, 2] = 0.0  # first numerator coefficient of the e-n Pade: u_en = 0
+        pk[k] = a
+    W = 1024
+    R0 = configs(pk, W, 63)
+    N = R0.shape[1]
+    tau = 0.003
+    print("case multidet DMC energy (Be)", flush=True)
+    with Context(pk, precision="fp64", max_walkers=3 * W, seed=12, replace_cusps=True,
+                 drift_mod=QMCB_DRIFT_UMRIGAR93_UNEQUAL, dt=tau) as ctx:
+        ctx.evaluate(np.arange(W), R0)
+        v = ctx.vmc_steps(W, 0, 300 * N, 0.01)
+        e_vmc = v.sum_e_local / v.n_samples
+        ctx.dmc_init(W)
+        st = ctx.dmc_steps(0, 400 * N, tau, e_vmc, target=W, reweight_after=0,
+                           energy_modification=True)
+        p = Property()
+        step = 400 * N
+        for blk in range(4):
+            st = ctx.dmc_steps(step, 500 * N, tau, st.e_trial, target=W, reweight_after=0,
+                               energy_modification=True)
+            step += 500 * N
+            series_energy(ctx.sample_series(), p)
+            print("block", blk, "population", st.population, "E_T", st.e_trial, flush=True)
+            assert 0.5 * W < st.population < 2.0 * W
+    s = p.summary()
+    print("VMC", e_vmc, "DMC", s.average, "+-", s.std_dev, flush=True)
+    assert s.decorr_depth >= 0 and s.std_dev < 1e-2
+    assert s.average < e_vmc - 0.1
+    assert abs(s.average - (-14.66736)) < 0.02
