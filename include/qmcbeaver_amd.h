@@ -273,13 +273,15 @@ qmcb_status qmcb_sync(qmcb_ctx *ctx);
 qmcb_status qmcb_set_walker_groups(qmcb_ctx *ctx, int32_t groups);
 
 /* ---- device memory ------------------------------------------------------
- * qmcb_device_bytes: device memory the context holds (requested sizes).
- * qmcb_plan_device_bytes: host-only projection of the most a context of
- * this system and these options allocates: qmcb_create plus the staging
- * arrays of the first single-electron qmcb_evaluate (or, with several
- * determinants, the first move) (cusp tables at
- * their upper bound; the per-sweep series buffer, 32 bytes per recorded
- * sweep, excluded), for admission control of many contexts on one device.
+ * qmcb_create makes one device allocation (an arena) holding every array of
+ * the context, including the staging arrays the first single-electron
+ * qmcb_evaluate (or, with several determinants, the first move) uses.
+ * qmcb_plan_device_bytes: host-only projection of that arena for this system
+ * and these options (cusp tables at their upper bound), for admission control
+ * of many contexts on one device.  qmcb_device_bytes: what the context holds,
+ * the arena plus the per-sweep series buffer (32 bytes per recorded sweep)
+ * and, once walker groups are used, their slot map (4 bytes per walker
+ * slot).
  * No reference counterpart: the reference sizes host memory per walker
  * (QMCWalker.cpp) and runs one molecule per process. */
 qmcb_status qmcb_device_bytes(const qmcb_ctx *ctx, int64_t *bytes);

@@ -56,7 +56,9 @@ struct qmcb_ctx {
   int Wc = 0;
   DevSystem dsys{};
   std::vector<void *> allocs;
-  size_t dev_bytes = 0;  // device memory requested by this context
+  size_t dev_bytes = 0;  // device memory this context holds (arena + own allocations)
+  char *arena = nullptr;
+  size_t arena_cap = 0, arena_used = 0;
   // single-electron orbital GEMM: waves per workgroup chosen by tune_gemm
   // (0 = not tuned yet) and the batch size it was tuned at, per spin
   int gemm_wpb[2] = {0, 0};
@@ -144,29 +146,40 @@ namespace {
     }                                                                           \
   } while (0)
 
-// Device allocations are zero-filled on the context's own stream (it is a
-// non-blocking stream: work on the legacy null stream would not be ordered
-// before its kernels); qmcb_create synchronises the stream before returning.
-template <typename T>
-hipError_t dalloc(qmcb_ctx *ctx, T **p, size_t count) {
-  void *q = nullptr;
-  size_t bytes = std::max<size_t>(count * sizeof(T), 16);
-  hipError_t e = hipMalloc(&q, bytes);
-  if (e != hipSuccess) return e;
-  ctx->allocs.push_back(q);
-  ctx->dev_bytes += bytes;
-  e = hipMemsetAsync(q, 0, bytes, ctx->stream);
-  *p = (T *)q;
-  return e;
+// Device memory.  qmcb_create makes one allocation, the arena, of the size
+// plan_device_bytes projects for the context, zero-fills it on the context's
+// own stream (a non-blocking stream: work on the legacy null stream would not
+// be ordered before its kernels) and carves every array out of it, each
+// rounded up to 256 bytes; qmcb_create synchronises the stream before
+// returning.  What the projection leaves out (the per-sweep series buffer,
+// the walker-group slot map of a context created without groups) gets an
+// allocation of its own, zero-filled the same way.
+constexpr size_t kArenaAlign = 256;
+inline size_t arena_round(size_t bytes) {
+  return (std::max<size_t>(bytes, 16) + kArenaAlign - 1) / kArenaAlign * kArenaAlign;
 }
 
-hipError_t dalloc_bytes(qmcb_ctx *ctx, void **p, size_t bytes) {
+hipError_t dalloc_bytes(qmcb_ctx *ctx, void **p, size_t bytes, bool own = false) {
+  const size_t r = arena_round(bytes);
+  if (!own && ctx->arena && ctx->arena_used + r <= ctx->arena_cap) {
+    *p = ctx->arena + ctx->arena_used;
+    ctx->arena_used += r;
+    return hipSuccess;
+  }
   bytes = std::max<size_t>(bytes, 16);
   hipError_t e = hipMalloc(p, bytes);
   if (e != hipSuccess) return e;
   ctx->allocs.push_back(*p);
   ctx->dev_bytes += bytes;
   return hipMemsetAsync(*p, 0, bytes, ctx->stream);
+}
+
+template <typename T>
+hipError_t dalloc(qmcb_ctx *ctx, T **p, size_t count, bool own = false) {
+  void *q = nullptr;
+  hipError_t e = dalloc_bytes(ctx, &q, count * sizeof(T), own);
+  *p = (T *)q;
+  return e;
 }
 
 // FP32 -> BF16 bits, round to nearest even (NaN kept quiet)
@@ -315,13 +328,12 @@ MdetArgs mdet_args(qmcb_ctx *ctx, int m, int mode, const qmcb_results &out) {
 // the CUs decides its speed.  The first move of a spin (and any later call
 // whose batch differs by more than 2x from the tuned one) times the 4-wave
 // and 2-wave workgroup variants on the real operands, three launches each,
-// and keeps the faster.  Both variants produce identical bits.
-#ifndef QMCB_GEMM_TUNE
-#define QMCB_GEMM_TUNE 1
-#endif
+// and keeps the faster; batches below 4096 rows (where the timing runs would
+// cost more than they can save) take the 4-wave variant.  Both variants
+// produce identical bits.
 template <typename F>
 qmcb_status tune_gemm(qmcb_ctx *ctx, int sp, int num, F &&gemm) {
-  if (!QMCB_GEMM_TUNE) {
+  if (num < 4096) {  // small batches: the 4-wave variant, no timing runs
     ctx->gemm_wpb[sp] = 4;
     ctx->gemm_tuned_num[sp] = num;
     return QMCB_OK;
@@ -603,6 +615,8 @@ hipError_t upload(qmcb_ctx *ctx, const T **dst, const T *src, size_t count) {
   return e;
 }
 
+size_t plan_device_bytes(const qmcb_system *s, const qmcb_options *o);
+
 qmcb_status create_impl(const qmcb_system *s, const qmcb_options *o, qmcb_ctx *ctx) {
   ctx->opt = *o;
   if (ctx->opt.dt <= 0) ctx->opt.dt = 0.1;
@@ -625,6 +639,14 @@ qmcb_status create_impl(const qmcb_system *s, const qmcb_options *o, qmcb_ctx *c
   QCHK(hipSetDevice(o->device));
   ctx->device = o->device;
   QCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  {
+    const size_t cap = plan_device_bytes(s, o);
+    QCHK(hipMalloc((void **)&ctx->arena, cap));
+    ctx->allocs.push_back(ctx->arena);
+    ctx->arena_cap = cap;
+    ctx->dev_bytes += cap;
+    QCHK(hipMemsetAsync(ctx->arena, 0, cap, ctx->stream));
+  }
 
 
   // ---- system tables
@@ -913,15 +935,15 @@ qmcb_status evaluate_core(qmcb_ctx *ctx, int num, const int *d_ids, const double
   return DISPATCH(ctx, sm(ctx, a, d_ids, num));
 }
 
-// The most device memory a context of this system and these options
-// requests: the arrays create_impl allocates plus the staging arrays
+// The device memory a context of this system and these options holds in its
+// arena: the arrays create_impl allocates plus the staging arrays
 // (ensure_staging: first single-electron evaluate, or first move with
-// several determinants), with the cusp tables at their upper bound (every primitive an s
+// several determinants), each rounded as the arena rounds them, with the cusp tables at their upper bound (every primitive an s
 // Gaussian).  The per-sweep series buffer (32 bytes per recorded sweep) is
 // not included.
 size_t plan_device_bytes(const qmcb_system *s, const qmcb_options *o) {
   size_t total = 0;
-  auto add = [&](size_t count, size_t elem) { total += std::max<size_t>(count * elem, 16); };
+  auto add = [&](size_t count, size_t elem) { total += arena_round(count * elem); };
   const bool f64 = o->precision == QMCB_FP64;
   const bool store32 = o->precision == QMCB_MIXED_F32STORE;
   const size_t ts = f64 ? 8 : 4, to = (f64 || store32) ? 8 : 4;
@@ -1023,7 +1045,6 @@ size_t plan_device_bytes(const qmcb_system *s, const qmcb_options *o) {
   add(WD, 4);
   add(W, 4);
   if (multi) add(W * N * 3, 8);
-  if (o->walker_groups > 1 && !multi) add(W, 4);  // slot map of the walker groups
   return total;
 }
 
@@ -1269,7 +1290,10 @@ static qmcb_status ensure_groups(qmcb_ctx *ctx, int G) {
   if (!ctx->d_iota) {
     std::vector<int> iota(ctx->Wc);
     for (int i = 0; i < ctx->Wc; i++) iota[i] = i;
-    QCHK(upload(ctx, (const int **)&ctx->d_iota, iota.data(), iota.size()));
+    QCHK(dalloc(ctx, &ctx->d_iota, iota.size(), true));
+    QCHK(hipMemcpyAsync(ctx->d_iota, iota.data(), iota.size() * sizeof(int),
+                        hipMemcpyHostToDevice, ctx->stream));
+    QCHK(hipStreamSynchronize(ctx->stream));  // iota is pageable host memory
   }
   if (!ctx->main_event) QCHK(hipEventCreateWithFlags(&ctx->main_event, hipEventDisableTiming));
   for (int g = 0; g < G; g++) {
@@ -1308,7 +1332,7 @@ static qmcb_status series_reserve(qmcb_ctx *ctx, int64_t first_step, int32_t n_s
   }
   if (rows > ctx->series_cap) {
     const int cap = (int)std::max<int64_t>(rows, 2 * (int64_t)ctx->series_cap);
-    QCHK(dalloc(ctx, &ctx->d_series, 4 * (size_t)cap));
+    QCHK(dalloc(ctx, &ctx->d_series, 4 * (size_t)cap, true));
     ctx->series_cap = cap;
   }
   return QMCB_OK;

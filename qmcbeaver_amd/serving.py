@@ -90,10 +90,11 @@ class Job:
 def poisson_trace(n_jobs: int, rate: float, seed: int, dmc_fraction: float = 0.25,
                   e_range=(8, 256), moves_range=(1e3, 1e5), block_steps: int = 100,
                   max_blocks: int = 4) -> List[Job]:
-    """Jobs arriving as a Poisson stream of `rate` per second.  Electron
-    counts are even and uniform on e_range; the requested walker-moves M are
-    log-uniform on moves_range; a job runs 1..max_blocks blocks of
-    block_steps steps and ceil(M / (block_steps * blocks)) walkers."""
+    """Jobs arriving as a Poisson stream of `rate` per second (rate = inf:
+    the same jobs all at t = 0).  Electron counts are even and uniform on
+    e_range; the requested walker-moves M are log-uniform on moves_range; a
+    job runs 1..max_blocks blocks of block_steps steps and
+    ceil(M / (block_steps * blocks)) walkers."""
     rng = np.random.default_rng(seed)
     t = 0.0
     jobs = []

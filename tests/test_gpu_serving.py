@@ -1,10 +1,11 @@
 """Many molecules on one device: footprint projection, isolation of jobs
 in the mix, and the serving loop's admission with real contexts.
 
-* qmcb_plan_device_bytes equals what a context holds after its first
-  single-electron evaluate (single determinant, every precision, several
-  determinants), plus 32 bytes per recorded sweep of the series, and
-  bounds it with cusp replacement (cusp tables projected at their maximum);
+* qmcb_plan_device_bytes equals the arena a context allocates at creation,
+  which holds every array including the staging of the first single-electron
+  evaluate (single determinant, every precision, several determinants), and
+  the context holds 32 bytes more per recorded sweep of the series (with
+  cusp replacement the arena reserves the cusp tables at their maximum);
 * isolation: every job's per-block results (VMC energy and acceptance sums,
   DMC weighted energy sums, populations, trial energies) are bit-identical
   when the job runs alone and when it runs in the mix with others on the
@@ -47,11 +48,11 @@ def test_plan_equals_allocation(precision, n_elec, W, scratch):
     s = synth.make_system(-1, seed=11 + n_elec, n_elec=n_elec)
     R = synth.initial_walkers(s, W, 3)
     with Context(p, precision=precision, max_walkers=W, basis_scratch_bytes=scratch) as c:
+        assert c.device_bytes() == plan  # the arena, allocated at creation
         c.evaluate(np.arange(W), R)
-        before = c.device_bytes()
         R[:, 0] += 0.1
-        c.evaluate(np.arange(W), R, whichE=0)  # allocates the staging arrays
-        assert before < c.device_bytes() == plan
+        c.evaluate(np.arange(W), R, whichE=0)  # staging arrays come out of the arena
+        assert c.device_bytes() == plan
         c.vmc_steps(W, 0, 2, 0.1)  # + one row of the per-sweep series (32 bytes)
         assert c.device_bytes() == plan + 32
 
@@ -66,10 +67,11 @@ def test_plan_multidet_and_cusps():
         c.vmc_steps(W, 0, 2, 0.1)  # several determinants stage moves; one series row
         assert c.device_bytes() == plan + 32
     plan_c = plan_device_bytes(p, precision="fp64", max_walkers=W, replace_cusps=True)
+    assert plan < plan_c
     with Context(p, precision="fp64", max_walkers=W, replace_cusps=True) as c:
         c.evaluate(np.arange(W), R)
         c.vmc_steps(W, 0, 2, 0.1)
-        assert plan <= c.device_bytes() <= plan_c
+        assert c.device_bytes() == plan_c + 32
 
 
 def _small_trace():
