@@ -160,34 +160,53 @@ def accuracy_report(s, n_conf=10000, seed=4321, tau=0.01, sweeps=1, precision="f
 
 def cpu_baseline(s, tau, seed, target_s=15.0):
     """The reference's own evaluate (oracle/_ref, compiled from its sources)
-    timed on this host's cores, 1 thread; else the C oracle port."""
+    timed on this host's cores: one single-threaded reference process per
+    core (at most 16, the host share of one GPU), each on its own walkers,
+    all started together; the rate is the sum of their walker-moves over the
+    slowest one's time.  Without the reference build: the C oracle port on
+    one core."""
     from qmcbeaver_amd import synth
     drv = os.path.join(ROOT, "oracle", "_ref", "qmcb_ref_driver")
     nw = 8
-    R = synth.initial_walkers(s, nw, seed)
     if os.path.exists(drv):
+        try:
+            cores = max(1, min(16, len(os.sched_getaffinity(0))))
+        except AttributeError:
+            cores = 1
         with tempfile.TemporaryDirectory() as tmp:
             ck = os.path.join(tmp, "in.ckmf")
             synth.write_ckmf(s, ck, dt=tau)
-            cf = os.path.join(tmp, "cfg.bin")
-            with open(cf, "wb") as f:
-                np.array([nw, s.n_elec], dtype=np.int32).tofile(f)
-                R.tofile(f)
+            cfs = []
+            for c in range(cores):
+                cf = os.path.join(tmp, f"cfg{c}.bin")
+                with open(cf, "wb") as f:
+                    np.array([nw, s.n_elec], dtype=np.int32).tofile(f)
+                    synth.initial_walkers(s, nw, seed + c).tofile(f)
+                cfs.append(cf)
             # calibrate on a short run, then a bounded sample of ~target_s
-            moves = 16
-            out = subprocess.run([drv, "vmc", ck, cf, str(moves), str(tau), "7"], cwd=tmp,
+            out = subprocess.run([drv, "vmc", ck, cfs[0], "16", str(tau), "7"], cwd=tmp,
                                  capture_output=True, text=True, check=True).stdout
             r = json.loads(out.strip().splitlines()[-1])
             rate = r["walker_moves"] / max(r["seconds"], 1e-9)
             moves = int(max(16, min(4096, target_s * rate / nw)))
-            out = subprocess.run([drv, "vmc", ck, cf, str(moves), str(tau), "7"], cwd=tmp,
-                                 capture_output=True, text=True, check=True).stdout
-            r = json.loads(out.strip().splitlines()[-1])
-        return {"value": r["walker_moves"] / r["seconds"], "unit": "walker-moves/s", "cores": 1,
+            procs = [subprocess.Popen([drv, "vmc", ck, cf, str(moves), str(tau), str(7 + c)],
+                                      cwd=tmp, stdout=subprocess.PIPE, stderr=subprocess.DEVNULL,
+                                      text=True) for c, cf in enumerate(cfs)]
+            res = []
+            for pr in procs:
+                o, _ = pr.communicate()
+                if pr.returncode != 0:
+                    raise RuntimeError("reference driver failed")
+                res.append(json.loads(o.strip().splitlines()[-1]))
+        total = sum(r["walker_moves"] for r in res)
+        slowest = max(r["seconds"] for r in res)
+        return {"value": total / slowest, "unit": "walker-moves/s", "cores": cores,
                 "kind": "reference",
-                "sample": f"{nw} walkers x {moves} single-electron VMC moves of the same molecule "
-                          f"(reference QMCSCFJastrow::evaluate, re-evaluate on rejection), "
-                          f"{r['seconds']:.1f} s"}
+                "per_core": res[0]["walker_moves"] / res[0]["seconds"],
+                "sample": f"{cores} processes x {nw} walkers x {moves} single-electron VMC moves "
+                          f"of the same molecule (reference QMCSCFJastrow::evaluate, "
+                          f"re-evaluate on rejection), slowest {slowest:.1f} s"}
+    R = synth.initial_walkers(s, nw, seed)
     from oracle.pyoracle import Oracle
     o = Oracle(s.pack())
     t, _ = o.vmc_time(R, 32, tau)
