@@ -153,3 +153,18 @@ def test_refresh_equals_fresh_evaluate():
         assert rel(res["e_local"], ref["e_local"]) < 1e-10
         assert rel(res["log_psi"], ref["log_psi"]) < 1e-10
         assert np.allclose(res["det_ratio"], 1.0, rtol=1e-12)
+
+
+@pytest.mark.parametrize("name,cfg", [("cfg0_h2", 0), ("cfg1_organic64", 1)])
+def test_umrigar93_equalelectrons_drift_matches_reference(name, cfg):
+    """The equal-electron drift modification against the reference's evaluate
+    (tests/golden/make_golden_drift_equal.py), all-electron, <= 1e-10."""
+    g = np.load(os.path.join(GOLDEN, "drift_equal.npz"))
+    s = synth.make_system(cfg)
+    R = g[name + "_R"]
+    say("case", name, "umrigar93 equal-electron drift")
+    with Context(s.pack(), precision="fp64", max_walkers=R.shape[0], drift_mod=2, dt=0.1,
+                 umrigar_a=0.5) as ctx:
+        res = ctx.evaluate(np.arange(R.shape[0]), R)
+    assert rel(res["mod_drift"], g[name + "_mod_drift"]) < 1e-10
+    assert rel(res["e_local"], g[name + "_e_local"]) < 1e-10

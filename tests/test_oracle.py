@@ -244,3 +244,20 @@ def test_coulomb_and_jastrow_brute_force():
             U += a * d[i, j] / (1 + d[i, j])
         U += np.sum(-1.0 * dA / (1 + dA))
         assert abs(res["jastrow_u"][w] - U) < 1e-10 * abs(U)
+
+
+@pytest.mark.parametrize("name,cfg", [("cfg0_h2", 0), ("cfg1_organic64", 1)])
+def test_golden_umrigar93_equalelectrons_drift(name, cfg):
+    """The equal-electron drift factor (one factor from |grad ln Psi|^2 of all
+    electrons, umrigar93_equalelectrons_parameter 0.5, dt 0.1) against the
+    reference's evaluate (tests/golden/make_golden_drift_equal.py)."""
+    g = np.load(os.path.join(GOLDEN, "drift_equal.npz"))
+    s = synth.make_system(cfg)
+    assert s.digest() == str(g[name + "_digest"])
+    o = Oracle(s.pack(), drift_mod=2, dt=0.1, umrigar_a=0.5)
+    res = o.evaluate_all(g[name + "_R"])
+    assert rel(res["grad_lnpsi"], g[name + "_grad_lnpsi"]) < 1e-10
+    assert rel(res["mod_drift"], g[name + "_mod_drift"]) < 1e-10
+    # the factor is one scalar per walker and below 1
+    f = g[name + "_mod_drift"] / g[name + "_grad_lnpsi"]
+    assert np.allclose(f, f[:, :1, :1], rtol=1e-9) and (f < 1).all()
