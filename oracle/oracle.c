@@ -6,7 +6,7 @@
  * cpu_baseline leg load it (as liboracle.so via ctypes).
  *
  * Each function cites the reference code it restates (paths relative to
- * QMcBeaver/).  Parity pins (tests/test_oracle.py, tests/test_golden.py):
+ * QMcBeaver/).  Parity pins (tests/test_oracle.py):
  *   orc_basis         golden vectors from the reference build (evaluate level),
  *                     closed form of an s Gaussian, finite differences of the
  *                     value for gradient and Laplacian.
@@ -17,7 +17,14 @@
  *                     QMCSCFJastrow::evaluate compiled from its sources), all-
  *                     electron and single-electron sequences; FD of ln|Psi|
  *                     for the gradient and Laplacian; hydrogen-like closed form.
+ *                     Both umrigar93 drift modifications against reference
+ *                     evaluates with those flags.
  *   orc_vmc_time      timing loop only (no parity claim beyond orc_evaluate).
+ *
+ * Scope: one determinant without cusp replacement.  Several determinants and
+ * the cusp replacement are pinned on the device directly against golden
+ * vectors of the reference build (tests/golden/make_golden_multidet.py,
+ * make_golden_cusp.py), and the cusp fit on the host (tests/test_cusp.py).
  *
  * Deliberate difference: the reference's exponent cache is not reset between
  * atoms (QMCBasisFunction.cpp:310,372); here it is reset per atom.  The input
