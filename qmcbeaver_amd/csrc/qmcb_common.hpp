@@ -107,10 +107,60 @@ __device__ inline double drift_factor_unequal(const double g[3], const double x[
   return drift_factor_near(g, x, nearest_nucleus(x, nn, nuc_xyz), nuc_xyz, nuc_z, dt);
 }
 
+// Cross-lane moves through DPP (data-parallel primitives: the VALU reads a
+// neighbouring lane's register in the same instruction) instead of the LDS
+// permute path: a few cycles per step instead of an LDS round trip.  Every
+// pattern used here has a valid source lane for every lane.
+//   0xB1 quad_perm [1,0,3,2]: lane ^ 1      0x4E quad_perm [2,3,0,1]: lane ^ 2
+//   0xA0 quad_perm [0,0,2,2]: even lane of each pair
+//   0x141 row_half_mirror: i <-> 7 - i in each group of 8
+//   0x140 row_mirror: i <-> 15 - i in each row of 16
+template <int CTRL>
+__device__ __forceinline__ int dpp_i32(int v) {
+  return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = dpp_i32<CTRL>((int)b), hi = dpp_i32<CTRL>((int)(b >> 32));
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_readlane((int)b, lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Sum over the 64 lanes of a wave (all lanes active); every lane gets it.
 __device__ __forceinline__ double wave_sum(double v) {
+  v += dpp_f64<0xB1>(v);
+  v += dpp_f64<0x4E>(v);
+  v += dpp_f64<0x141>(v);
+  v += dpp_f64<0x140>(v);  // each lane: the sum of its row of 16
+  return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
+}
+
+// Largest value over the 64 lanes with the lowest index among equals (all
+// lanes active); every lane gets the winner.
+__device__ __forceinline__ void wave_argmax(double &best, int &bi) {
+  auto take = [&](double ob, int oi) {
+    if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+  };
+  take(dpp_f64<0xB1>(best), dpp_i32<0xB1>(bi));
+  take(dpp_f64<0x4E>(best), dpp_i32<0x4E>(bi));
+  take(dpp_f64<0x141>(best), dpp_i32<0x141>(bi));
+  take(dpp_f64<0x140>(best), dpp_i32<0x140>(bi));
+  double b0 = readlane_f64(best, 0);
+  int i0 = __builtin_amdgcn_readlane(bi, 0);
 #pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+  for (int r = 16; r < 64; r += 16) {
+    const double br = readlane_f64(best, r);
+    const int ir = __builtin_amdgcn_readlane(bi, r);
+    if (br > b0 || (br == b0 && ir < i0)) { b0 = br; i0 = ir; }
+  }
+  best = b0;
+  bi = i0;
 }
 
 // ---- Philox4x32-10 (Salmon et al., SC'11) counter-based RNG ---------------

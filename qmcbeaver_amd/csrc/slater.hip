@@ -227,15 +227,10 @@ gj_rows_kernel(SlaterArgs A, const int *walkers, int num) {
     for (int kk = 0; kk < KB; kk++) {
       const int k = kb * KB + kk;
       if (k < n && !sing) {
-        const double akk = __shfl_xor(a[kk], h, 64);  // position kk of half 0
+        const double akk = dpp_f64<0xA0>(a[kk]);  // position kk of half 0
         double best = (h == 0 && i >= k && i < n) ? fabs(akk) : -1.0;
         int bi = i;
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) {
-          const double ob = __shfl_xor(best, off, 64);
-          const int oi = __shfl_xor(bi, off, 64);
-          if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
-        }
+        wave_argmax(best, bi);
         if ((t & 63) == 0) { wmax[t >> 6] = best; widx[t >> 6] = bi; }
         __syncthreads();
         best = wmax[0];
@@ -273,7 +268,7 @@ gj_rows_kernel(SlaterArgs A, const int *walkers, int num) {
           mant = frexp(mant * fabs(piv), &e2);
           expo += e2;
           const double ip = 1.0 / piv;
-          const double f = __shfl_xor(a[kk], h, 64);  // after the interchange
+          const double f = dpp_f64<0xA0>(a[kk]);  // after the interchange
           if (i == k) {
 #pragma unroll
             for (int j = 0; j < H; j++) a[j] = (h == 0 && j == kk) ? ip : prow[h * H + j] * ip;
@@ -291,7 +286,7 @@ gj_rows_kernel(SlaterArgs A, const int *walkers, int num) {
     // plus half 1's [0, KB); half 1 takes its own [KB, H) plus half 0's [0, KB)
     double tq[KB];
 #pragma unroll
-    for (int q = 0; q < KB; q++) tq[q] = __shfl_xor(a[q], 1, 64);
+    for (int q = 0; q < KB; q++) tq[q] = dpp_f64<0xB1>(a[q]);  // partner lane (lane ^ 1)
 #pragma unroll
     for (int j = 0; j < H - KB; j++) a[j] = a[j + KB];
 #pragma unroll
