@@ -141,9 +141,9 @@ __device__ __forceinline__ double wave_sum(double v) {
   return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
 }
 
-// Largest value over the 64 lanes with the lowest index among equals (all
-// lanes active); every lane gets the winner.
-__device__ __forceinline__ void wave_argmax(double &best, int &bi) {
+// Largest value over each row of 16 lanes with the lowest index among
+// equals (the row's lanes active); every lane of the row gets the winner.
+__device__ __forceinline__ void row16_argmax(double &best, int &bi) {
   auto take = [&](double ob, int oi) {
     if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
   };
@@ -151,6 +151,11 @@ __device__ __forceinline__ void wave_argmax(double &best, int &bi) {
   take(dpp_f64<0x4E>(best), dpp_i32<0x4E>(bi));
   take(dpp_f64<0x141>(best), dpp_i32<0x141>(bi));
   take(dpp_f64<0x140>(best), dpp_i32<0x140>(bi));
+}
+
+// The same over the 64 lanes (all lanes active); every lane gets the winner.
+__device__ __forceinline__ void wave_argmax(double &best, int &bi) {
+  row16_argmax(best, bi);
   double b0 = readlane_f64(best, 0);
   int i0 = __builtin_amdgcn_readlane(bi, 0);
 #pragma unroll

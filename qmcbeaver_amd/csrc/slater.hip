@@ -81,11 +81,15 @@ lu_inverse_kernel(SlaterArgs A, const int *walkers, int num) {
           if (v > best) { best = v; bi = gi; }
         }
       }
+      if constexpr (G == 16) {
+        row16_argmax(best, bi);  // the owners are one DPP row of 16 lanes
+      } else {
 #pragma unroll
-      for (int off = G / 2; off >= 1; off >>= 1) {
-        const double ob = __shfl_xor(best, off, 64);
-        const int oi = __shfl_xor(bi, off, 64);
-        if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+        for (int off = G / 2; off >= 1; off >>= 1) {
+          const double ob = __shfl_xor(best, off, 64);
+          const int oi = __shfl_xor(bi, off, 64);
+          if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+        }
       }
       if (tr == 0) { s_piv[buf] = bi; s_max[buf] = best; }
     }
