@@ -7,10 +7,15 @@ proposes a move of electron (step mod 256), the full evaluate contract runs at
 the proposal (ln|Psi|, sign, grad ln Psi of all electrons, Laplacian, E_L) and
 the move is accepted or rejected on the device.  The mixed path (basis + MFMA
 orbital GEMM in FP32, Sherman-Morrison inverse/log-det/E_L in FP64, FP64
-re-inversion every 64 moves) is `value`; the plain FP64 path on the same GPU
-is reported beside it.
+re-inversion every 64 moves), with the walkers in 3 overlapping groups, is
+`value`; the plain FP64 path on the same GPU, the fp32store and bf16 modes,
+their per-quantity errors against the FP64 path, a serving run of 200
+Poisson-arriving jobs and the reference's own CPU evaluate on the host cores
+are reported beside it.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py --workload batched     # uneven DMC over 64 molecules
+    python bench.py --workload serving     # continuous-batching serving loop
 """
 from __future__ import annotations
 
