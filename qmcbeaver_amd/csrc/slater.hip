@@ -425,9 +425,26 @@ hipError_t launch_traces_all(const SlaterArgs &a, const int *walkers, int num, h
 // Pass 1 streams A_0 and the four derivative matrices once, forms the current
 // inverse element on the fly and accumulates w, T1, T2 per column (9 registers).
 // Thread mapping: c = t % NC (electron column), sl = t / NC (row slice).
-#ifndef QMCB_SM_UNROLL
-#define QMCB_SM_UNROLL 2
+// Rows per software-pipeline stage of the read-only stream (2U rows in
+// flight per lane).  For the 128-column tile the deeper pipelines raise the
+// register count above 128 (FP32 derivatives: 131 VGPRs, 3 waves per SIMD;
+// FP64: 213, 2 waves per SIMD) with the streaming rate unchanged, which frees
+// wave slots for the basis, Jastrow and GEMM kernels of the other walker
+// groups running beside sm_move (+7 % / +8 % walker-moves/s with three groups
+// at 256 electrons, equal with one group).
+#ifndef QMCB_SM_UNROLL_F32
+#define QMCB_SM_UNROLL_F32 4
 #endif
+#ifndef QMCB_SM_UNROLL_F64
+#define QMCB_SM_UNROLL_F64 6
+#endif
+#ifndef QMCB_SM_UNROLL_SMALL
+#define QMCB_SM_UNROLL_SMALL 2
+#endif
+template <typename TS, int NC>
+__host__ __device__ constexpr int sm_unroll() {
+  return NC < 128 ? QMCB_SM_UNROLL_SMALL : (sizeof(TS) == 4 ? QMCB_SM_UNROLL_F32 : QMCB_SM_UNROLL_F64);
+}
 #ifndef QMCB_SM_MINWAVES
 #define QMCB_SM_MINWAVES 1
 #endif
@@ -531,7 +548,7 @@ sm_move_kernel(MoveArgs A, const int *walkers, int num) {
       // read-only stream, software-pipelined in registers: the loads of the
       // next U rows are issued before the current U rows are consumed, so
       // every wave keeps 2U rows (U x 24/40 bytes per lane) in flight
-      constexpr int U = QMCB_SM_UNROLL;
+      constexpr int U = sm_unroll<TS, NC>();
       const double *__restrict__ ip = inv + c;
       double ca[U], na[U];
       TS cx[U], cy[U], cz[U], cl[U], nx[U], ny[U], nz[U], nl[U];
