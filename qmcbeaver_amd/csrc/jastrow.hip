@@ -113,7 +113,13 @@ hipError_t launch_jastrow_all(const JastrowArgs &a, const int *walkers, int num,
 // Writes U', grad U' for all N electrons, Lap U', V_ee', V_en' to batch row b.
 constexpr int kMT = 128;
 
-__global__ void __launch_bounds__(kMT)
+// At most 76 VGPRs (6 waves per SIMD, no spills): 10 % faster alone than at
+// its free register count (122, 4 waves), and its waves fit beside sm_move's
+// (3 or 2 waves per SIMD) when walker groups overlap the two kernels.
+#ifndef QMCB_JMOVE_WAVES
+#define QMCB_JMOVE_WAVES 6
+#endif
+__global__ void __launch_bounds__(kMT) __attribute__((amdgpu_waves_per_eu(QMCB_JMOVE_WAVES, 8)))
 jastrow_move_kernel(JastrowArgs A, const double *cU, const double *cUx, const double *cUxx,
                     const double *cVee, const double *cVen, const int *walkers, int num) {
   const int b = blockIdx.x;
