@@ -226,27 +226,31 @@ orbital_gemm_kernel(const TI *__restrict__ Ct, int ldc, int n_orb, const TI *__r
   }
 }
 
-// Orbital tiles of at most 16 * kSplitMT columns; wider orbital sets split over
-// grid.z (each workgroup re-reads its points' X rows, mostly from L2).  At 128
-// orbitals two 64-column tiles beat one 128-column tile even alone (FP32 155
-// against 162 us, FP64 265 against 316 us per 8192-row GEMM): twice the
-// workgroups at 76 / 138 registers and 20 / 37 KB of LDS each, against 130 /
-// 220 and 41 / 78 KB, and their waves fit beside sm_move's.
-#ifndef QMCB_GEMM_SPLIT_MT
-#define QMCB_GEMM_SPLIT_MT 4
-#endif
+// Orbital tiles.  Batches of up to kSplitRows points (single-electron moves:
+// one row per walker) use tiles of 64 orbital columns, wider orbital sets
+// split over grid.z, and each workgroup re-reads its points' X rows, mostly
+// from L2.  At 128 orbitals and 8192 rows two 64-column tiles beat one
+// 128-column tile even alone (FP32 155 against 162 us, FP64 260 against
+// 316 us): twice the workgroups at 76 / 138 registers and 20 / 37 KB of LDS
+// each, against 130 / 220 and 41 / 78 KB, and their waves fit beside
+// sm_move's.  Larger batches (the all-electron evaluate, ~10^5 rows per
+// launch) stream X from HBM, where reading it once with 128-column tiles is
+// faster (1418 against 1702 us FP32 per 88320-row chunk).
+constexpr long kSplitRows = 16384;
 template <typename TI, typename TO, int WPB>
 static void gemm_grid(const TI *Ct, int ldc, int n_orb, const TI *X, int ldk, long ldp, long P,
                       int kpad, const GemmOut &o, hipStream_t st) {
-  constexpr int kSplitMT = QMCB_GEMM_SPLIT_MT, OW = 16 * kSplitMT;
-  static_assert(kSplitMT == 2 || kSplitMT == 4, "orbital tile of 32 or 64 columns");
   const unsigned gx = (unsigned)((P + 16 * WPB - 1) / (16 * WPB));
   const dim3 blk(64 * WPB);
-  if (n_orb > OW / 2) {
-    hipLaunchKernelGGL((orbital_gemm_kernel<TI, TO, kSplitMT, WPB>),
-                       dim3(gx, kComp, (unsigned)((n_orb + OW - 1) / OW)), blk, 0, st, Ct, ldc,
+  if (n_orb > 64 && P > kSplitRows) {
+    hipLaunchKernelGGL((orbital_gemm_kernel<TI, TO, 8, WPB>),
+                       dim3(gx, kComp, (unsigned)((n_orb + 127) / 128)), blk, 0, st, Ct, ldc,
                        n_orb, X, ldk, ldp, P, kpad, o);
-  } else if (n_orb > 16 && kSplitMT > 2) {
+  } else if (n_orb > 32) {
+    hipLaunchKernelGGL((orbital_gemm_kernel<TI, TO, 4, WPB>),
+                       dim3(gx, kComp, (unsigned)((n_orb + 63) / 64)), blk, 0, st, Ct, ldc, n_orb,
+                       X, ldk, ldp, P, kpad, o);
+  } else if (n_orb > 16) {
     hipLaunchKernelGGL((orbital_gemm_kernel<TI, TO, 2, WPB>), dim3(gx, kComp, 1), blk, 0, st, Ct,
                        ldc, n_orb, X, ldk, ldp, P, kpad, o);
   } else {
