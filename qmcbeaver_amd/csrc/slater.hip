@@ -462,6 +462,9 @@ __host__ __device__ constexpr int sm_unroll() {
 #ifndef QMCB_SM_NT64
 #define QMCB_SM_NT64 128
 #endif
+#ifndef QMCB_SM_TILE96
+#define QMCB_SM_TILE96 1
+#endif
 
 // streamed-once operands: optionally non-temporal loads
 template <typename T>
@@ -945,6 +948,9 @@ hipError_t launch_sm_move(const MoveArgs &a, const int *walkers, int num, hipStr
   else if (n <= 64)
     hipLaunchKernelGGL((sm_move_kernel<TS, TO, 64, QMCB_SM_NT64>), dim3(num), dim3(QMCB_SM_NT64),
                        0, st, a, walkers, num);
+  else if (n <= 96 && QMCB_SM_TILE96)
+    hipLaunchKernelGGL((sm_move_kernel<TS, TO, 96, 192>), dim3(num), dim3(192), 0, st, a,
+                       walkers, num);
   else if (n <= 128)
     hipLaunchKernelGGL((sm_move_kernel<TS, TO, 128, 256>), dim3(num), dim3(256), 0, st, a,
                        walkers, num);

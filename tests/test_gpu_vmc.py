@@ -59,6 +59,35 @@ def test_vmc_resident_state_matches_fresh_evaluate(precision, refresh, tol):
             ctx.accept(ids, np.zeros(W, dtype=np.uint8))
 
 
+@pytest.mark.parametrize("n_elec", [152, 200], ids=["n76_tile96", "n100_tile128"])
+@pytest.mark.parametrize("precision,refresh,tol", [("fp64", 0, 1e-8), ("fp32", 64, 1e-6)])
+def test_vmc_resident_state_column_tiles(n_elec, precision, refresh, tol):
+    # the same check on the 96- and 128-column Sherman-Morrison tiles
+    s = synth.make_system(-1, seed=5, n_elec=n_elec)
+    W, steps = 32, 200
+    say("vmc consistency", precision, "electrons", n_elec, "walkers", W, "steps", steps)
+    R0 = synth.initial_walkers(s, W, 23)
+    with Context(s.pack(), precision=precision, max_walkers=W, refresh_interval=refresh,
+                 seed=78) as ctx, \
+            Context(s.pack(), precision=precision, max_walkers=W) as fresh:
+        ids = np.arange(W)
+        ctx.evaluate(ids, R0)
+        st = ctx.vmc_steps(W, 0, steps, 0.01)
+        acc = st.sum_accept / (W * steps)
+        say("acceptance", acc)
+        assert 0.2 < acc < 1.0
+        R = ctx.positions(W)
+        ref = fresh.evaluate(ids, R)
+        assert not ref["singular"].any()
+        for e in (0, s.n_up - 1, s.n_up, s.n_elec - 1):
+            res = ctx.evaluate(ids, R, whichE=e)
+            assert np.allclose(res["det_ratio"], 1.0, rtol=1e-9, atol=0), e
+            assert rel(res["e_local"], ref["e_local"]) < tol, (e, rel(res["e_local"], ref["e_local"]))
+            assert rel(res["log_psi"], ref["log_psi"]) < tol
+            assert rel(res["grad_lnpsi"], ref["grad_lnpsi"]) < tol
+            ctx.accept(ids, np.zeros(W, dtype=np.uint8))
+
+
 def test_vmc_walkers_independent_of_batch():
     s = synth.make_system(1)
     say("vmc batch independence")
