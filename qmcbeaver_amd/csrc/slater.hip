@@ -209,9 +209,13 @@ gj_rows_kernel(SlaterArgs A, const int *walkers, int num) {
   const int w = w0 * D + d;  // this determinant's record
   const int n = A.n, nact = act_count(A);
   const int t = threadIdx.x, i = t >> 1, h = t & 1;
-  __shared__ double prow[NP], krow[NP];
-  __shared__ double wmax[NWV];
-  __shared__ int widx[NWV];
+  // pivot rows and arg-max slots alternate between two buffers by pivot
+  // parity: pivot k+1 writes the buffer pivot k is not reading, and pivot k+2
+  // rewrites pivot k's buffer only after every wave has passed both barriers
+  // of pivot k+1, so one barrier per pivot is saved
+  __shared__ double prow_b[2][NP], krow_b[2][NP];
+  __shared__ double wmax_b[2][NWV];
+  __shared__ int widx_b[2][NWV];
   __shared__ int ipiv[NP], dest[NP], cur[NP];
   __shared__ double tile[TR][NP + 1];
 
@@ -231,6 +235,8 @@ gj_rows_kernel(SlaterArgs A, const int *walkers, int num) {
     for (int kk = 0; kk < KB; kk++) {
       const int k = kb * KB + kk;
       if (k < n && !sing) {
+        double *prow = prow_b[k & 1], *krow = krow_b[k & 1], *wmax = wmax_b[k & 1];
+        int *widx = widx_b[k & 1];
         const double akk = dpp_f64<0xA0>(a[kk]);  // position kk of half 0
         double best = (h == 0 && i >= k && i < n) ? fabs(akk) : -1.0;
         int bi = i;
@@ -283,7 +289,6 @@ gj_rows_kernel(SlaterArgs A, const int *walkers, int num) {
               a[j] = (h == 0 && j == kk) ? -fi : a[j] - fi * prow[h * H + j];
           }
         }
-        __syncthreads();
       }
     }
     // rotate every row left by KB positions: half 0 takes its own [KB, H)
