@@ -1,7 +1,7 @@
 /*
  * oracle.c -- CPU restatement (plain C99) of QMcBeaver's FP64 evaluate path.
  *
- * TEST INFRASTRUCTURE ONLY: the checker for the HIP path.  Never linked into
+ * TEST INFRASTRUCTURE ONLY: the oracle the HIP path is tested against.  Never linked into
  * the product library; only tests/, __graft_entry__.smoke() and bench.py's
  * cpu_baseline leg load it (as liboracle.so via ctypes).
  *

@@ -1,7 +1,7 @@
 /*
  * oracle.h -- CPU restatement of QMcBeaver's FP64 walker evaluation.
  *
- * TEST INFRASTRUCTURE ONLY.  This is the checker the HIP path is compared
+ * TEST INFRASTRUCTURE ONLY.  This is the oracle the HIP path is compared
  * against; it is never linked into, loaded by or called from the product
  * library (qmcbeaver_amd/).  Only tests/, __graft_entry__.smoke() and the
  * cpu_baseline leg of bench.py may use it.

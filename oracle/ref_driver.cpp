@@ -87,7 +87,7 @@ static double now() {
   return tv.tv_sec + 1e-6 * tv.tv_usec;
 }
 
-// splitmix64 + Box-Muller: the driver's own move generator (the reference's
+// splitmix64 + Box-Muller: this program's own move generator (the reference's
 // ran1/gasdev streams are not part of the evaluate path).
 static unsigned long long g_state;
 static double unif() {

@@ -1073,7 +1073,7 @@ qmcb_status qmcb_create(const qmcb_system *sys, const qmcb_options *opt, qmcb_ct
   }
   if (opt->precision != QMCB_FP64 && opt->precision != QMCB_MIXED_FP32 &&
       opt->precision != QMCB_MIXED_F32STORE && opt->precision != QMCB_MIXED_BF16) {
-    g_create_error = "precision not supported by this build";
+    g_create_error = "precision not supported by this library";
     return QMCB_INVALID_ARG;
   }
   for (int j = 0; j < sys->n_basis; j++) {
