@@ -524,23 +524,51 @@ sm_move_kernel(MoveArgs A, const int *walkers, int num) {
   double zc[KD];
 #pragma unroll
   for (int a = 0; a < KD; a++) zc[a] = (a < dc && c < n) ? zbw[(long)a * n + c] : 0.0;
+  // column r of A_0 (one element per thread: NT >= NC >= n), loaded with the
+  // first batch instead of after its barrier
+  static_assert(NT >= NC, "one column element per thread");
+  const double xr = t < n ? inv[(long)t * n + r] : 0.0;
   __syncthreads();
-  for (int o = t; o < n; o += NT) {
-    double x = inv[(long)o * n + r];
+
+  // ---- pass 1 set-up: the read-only stream's first U rows are requested
+  // before column r is formed, so their latency overlaps it and the barrier
+  const TS *dm = (const TS *)A.s.dmat + (long)w * kDer * nn2;
+  const TS *__restrict__ d1 = dm + c;
+  const TS *__restrict__ d2 = dm + 1 * nn2 + c;
+  const TS *__restrict__ d3 = dm + 2 * nn2 + c;
+  const TS *__restrict__ d4 = dm + 3 * nn2 + c;
+  // software pipeline in registers: the loads of the next U rows are issued
+  // before the current U rows are consumed, so every wave keeps 2U rows
+  // (U x 24/40 bytes per lane) in flight
+  constexpr int U = sm_unroll<TS, NC>();
+  const double *__restrict__ ip_s = inv + c;
+  double ca[U], na[U];
+  TS cx[U], cy[U], cz[U], cl[U], nx[U], ny[U], nz[U], nl[U];
+  auto load = [&](int o0, double *a, TS *x, TS *y, TS *z, TS *l) {
 #pragma unroll
-    for (int a = 0; a < KD; a++) x -= ub[a][o] * zr[a];
-    colr[o] = x;
+    for (int u = 0; u < U; u++) {
+      const int o = min(o0 + u, o_end - 1);  // clamped: in bounds, unused past o_end
+      const long off = (long)o * n;
+      a[u] = ld_stream(ip_s + off);
+      x[u] = ld_stream(d1 + off);
+      y[u] = ld_stream(d2 + off);
+      z[u] = ld_stream(d3 + off);
+      l[u] = ld_stream(d4 + off);
+    }
+  };
+  const bool stream = !flush && c < n;
+  if (stream && o_beg < o_end) load(o_beg, ca, cx, cy, cz, cl);
+  if (t < n) {
+    double x = xr;
+#pragma unroll
+    for (int a = 0; a < KD; a++) x -= ub[a][t] * zr[a];
+    colr[t] = x;
   }
   __syncthreads();
 
   // ---- pass 1
-  const TS *dm = (const TS *)A.s.dmat + (long)w * kDer * nn2;
   double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // w, T1[4], T2[4]
   if (c < n) {
-    const TS *__restrict__ d1 = dm + c;
-    const TS *__restrict__ d2 = dm + 1 * nn2 + c;
-    const TS *__restrict__ d3 = dm + 2 * nn2 + c;
-    const TS *__restrict__ d4 = dm + 3 * nn2 + c;
     // one row of the pass: the current inverse element from A_0 and the
     // pending terms, then the nine accumulators
     auto row = [&](int o, double a, double x, double y, double z, double l) {
@@ -552,27 +580,8 @@ sm_move_kernel(MoveArgs A, const int *walkers, int num) {
       acc[5] += x * cr; acc[6] += y * cr; acc[7] += z * cr; acc[8] += l * cr;
       return a;
     };
-    if (!flush) {
-      // read-only stream, software-pipelined in registers: the loads of the
-      // next U rows are issued before the current U rows are consumed, so
-      // every wave keeps 2U rows (U x 24/40 bytes per lane) in flight
-      constexpr int U = sm_unroll<TS, NC>();
-      const double *__restrict__ ip = inv + c;
-      double ca[U], na[U];
-      TS cx[U], cy[U], cz[U], cl[U], nx[U], ny[U], nz[U], nl[U];
-      auto load = [&](int o0, double *a, TS *x, TS *y, TS *z, TS *l) {
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-          const int o = min(o0 + u, o_end - 1);  // clamped: in bounds, unused past o_end
-          const long off = (long)o * n;
-          a[u] = ld_stream(ip + off);
-          x[u] = ld_stream(d1 + off);
-          y[u] = ld_stream(d2 + off);
-          z[u] = ld_stream(d3 + off);
-          l[u] = ld_stream(d4 + off);
-        }
-      };
-      if (o_beg < o_end) load(o_beg, ca, cx, cy, cz, cl);
+    if (stream) {
+      // read-only stream (its first U rows are already requested)
       for (int o0 = o_beg; o0 < o_end; o0 += U) {
         if (o0 + U < o_end) load(o0 + U, na, nx, ny, nz, nl);
 #pragma unroll
