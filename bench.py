@@ -486,6 +486,7 @@ def main():
             "walker_moves_per_s": srv["walker_moves_per_s"], "wall_s": srv["wall_s"],
             "completed": srv["completed"], "failed": srv["failed"], "rejected": srv["rejected"],
             "peak_active": srv["peak_active"], "peak_bytes": srv["peak_bytes"],
+            "warmup_jobs": srv["warmup_jobs"],
             "fp64": None if srv64 is None else {
                 "block_latency_ms": srv64["block_latency_ms"],
                 "walker_moves_per_s": srv64["walker_moves_per_s"], "wall_s": srv64["wall_s"],
