@@ -99,6 +99,7 @@ def run_path(s, precision, W, steps, warmup, tau, block, refresh, seed, groups=1
         done += k
     ctx.sync()
     dt = time.perf_counter() - t0
+    groups_used = ctx.walker_groups()
     ctx.set_walker_groups(1)
     ctx.enable_timing(True)
     tk = time.perf_counter()
@@ -118,6 +119,7 @@ def run_path(s, precision, W, steps, warmup, tau, block, refresh, seed, groups=1
         "init_s": t_init,
         "init_singular": n_sing,
         "kernels": kt,
+        "walker_groups": groups_used,
     }
 
 
@@ -289,7 +291,8 @@ def main():
     ap.add_argument("--block", type=int, default=16, help="steps per timed block")
     ap.add_argument("--refresh", type=int, default=64)
     ap.add_argument("--groups", type=int, default=3,
-                    help="walker groups on overlapping streams (results are bit-identical)")
+                    help="walker groups on overlapping streams (results are bit-identical); "
+                         "-1: chosen by the library's timing")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp32store", "bf16", "fp64"])
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -417,7 +420,7 @@ def main():
                          f"FP64 re-inversion every {args.refresh} moves"
                          if args.precision != "fp64" else "FP64 throughout",
             "walkers": args.walkers, "electrons": N, "basis_functions": s.n_basis,
-            "walker_groups": args.groups,
+            "walker_groups": mixed["walker_groups"],
         },
         "fp64": None if fp64 is None else {
             "value": fp64["walker_moves_per_s"], "ms_per_step": fp64["ms_per_step"],

@@ -125,7 +125,13 @@ typedef struct qmcb_options {
                                so that their kernels overlap; results are
                                bit-identical for every value.  0 or 1 = one
                                group; 3 is fastest at 8192 walkers of 256
-                               electrons (12 % over one group)                */
+                               electrons (12 % over one group).  -1:
+                               qmcb_vmc_steps times 16 steps of each of 1, 2
+                               and 3 groups (windows without a re-inversion,
+                               over as many calls as needed, one group until
+                               then) and keeps the fastest, again when the
+                               batch changes by more than 2x; qmcb_dmc_steps
+                               uses that choice                              */
 } qmcb_options;
 
 /* Caller-owned outputs, one row per evaluated walker.  Any pointer may be
@@ -270,9 +276,14 @@ qmcb_status qmcb_dmc_population(qmcb_ctx *ctx, int32_t cap, int32_t *slots,
 
 qmcb_status qmcb_sync(qmcb_ctx *ctx);
 
-/* Change qmcb_options.walker_groups of a live context (0..8); takes effect
+/* Change qmcb_options.walker_groups of a live context (-1..8); takes effect
  * at the next qmcb_vmc_steps.  No reference counterpart (a scheduling knob). */
 qmcb_status qmcb_set_walker_groups(qmcb_ctx *ctx, int32_t groups);
+
+/* The walker-group count in use: the option, or with walker_groups = -1 the
+ * count chosen by timing (0 while none is chosen yet).  No reference
+ * counterpart (a scheduling knob). */
+qmcb_status qmcb_get_walker_groups(const qmcb_ctx *ctx, int32_t *groups);
 
 /* ---- device memory ------------------------------------------------------
  * qmcb_create makes one device allocation (an arena) holding every array of

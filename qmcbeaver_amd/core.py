@@ -27,7 +27,8 @@ EXPORTS = ["qmcb_abi_version", "qmcb_create", "qmcb_destroy", "qmcb_last_error",
            "qmcb_property_destroy", "qmcb_property_zero", "qmcb_property_add",
            "qmcb_property_add_series", "qmcb_property_merge", "qmcb_property_reweight",
            "qmcb_property_summary_get", "qmcb_property_level", "qmcb_cusp_fit",
-           "qmcb_device_bytes", "qmcb_plan_device_bytes", "qmcb_set_walker_groups"]
+           "qmcb_device_bytes", "qmcb_plan_device_bytes", "qmcb_set_walker_groups",
+           "qmcb_get_walker_groups"]
 
 _lib = None
 
@@ -84,6 +85,8 @@ def lib():
     L.qmcb_cusp_fit.argtypes = [C.POINTER(SystemStruct), C.c_int32, C.POINTER(CuspOrbital)]
     L.qmcb_set_walker_groups.restype = C.c_int
     L.qmcb_set_walker_groups.argtypes = [vp, C.c_int32]
+    L.qmcb_get_walker_groups.restype = C.c_int
+    L.qmcb_get_walker_groups.argtypes = [vp, C.POINTER(C.c_int32)]
     L.qmcb_device_bytes.restype = C.c_int
     L.qmcb_device_bytes.argtypes = [vp, C.POINTER(C.c_int64)]
     L.qmcb_plan_device_bytes.restype = C.c_int
@@ -286,8 +289,15 @@ class Context:
         self._check(lib().qmcb_sync(self._h), "qmcb_sync")
 
     def set_walker_groups(self, groups: int):
-        """Walker groups of the next vmc_steps calls (results do not change)."""
+        """Walker groups of the next vmc_steps calls (results do not change);
+        -1: chosen by timing at the next call with enough steps."""
         self._check(lib().qmcb_set_walker_groups(self._h, int(groups)), "qmcb_set_walker_groups")
+
+    def walker_groups(self) -> int:
+        """Walker-group count in use (with -1: the timed choice, 0 before one)."""
+        g = C.c_int32(0)
+        self._check(lib().qmcb_get_walker_groups(self._h, C.byref(g)), "qmcb_get_walker_groups")
+        return int(g.value)
 
     def device_bytes(self) -> int:
         """Device memory this context holds."""

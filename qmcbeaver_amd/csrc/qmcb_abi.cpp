@@ -129,6 +129,10 @@ struct qmcb_ctx {
   std::map<int, int> staged_row;
   std::vector<int> initialized;  // per slot: has a resident state
   long sm_updates[2] = {0, 0};   // SM updates per spin since the last re-inversion
+  int auto_groups = 0;           // walker_groups < 0: the count chosen by timing (0: not yet)
+  int auto_groups_num = 0;       // ... for this many walkers
+  int tune_num = 0, tune_q = 0;  // timing in progress: batch size, candidates timed so far
+  float tune_ms[3] = {0, 0, 0};
   // timing
   bool timing = false;
   Timing tm;
@@ -1362,15 +1366,6 @@ qmcb_status qmcb_vmc_steps(qmcb_ctx *ctx, int32_t num, int64_t first_step, int32
     }
   }
   qmcb_results none{};
-  // walker groups: single determinant only; at least 256 walkers per group
-  int G = ctx->multi ? 1 : std::max(1, std::min<int>(ctx->opt.walker_groups, qmcb_ctx::kMaxGroups));
-  while (G > 1 && num / G < 256) G--;
-  if (G > 1) {
-    qmcb_status gs = ensure_groups(ctx, G);
-    if (gs != QMCB_OK) return gs;
-    gs = groups_wait_main(ctx, G);
-    if (gs != QMCB_OK) return gs;
-  }
   auto propose = [&](long step, int m, const int *d_map, int cnt) -> qmcb_status {
     ProposeArgs pa{};
     pa.N = ctx->N;
@@ -1391,8 +1386,16 @@ qmcb_status qmcb_vmc_steps(qmcb_ctx *ctx, int32_t num, int64_t first_step, int32
     TIMED("propose", launch_propose(pa, d_map, cnt, ctx->stream));
     return QMCB_OK;
   };
-  for (int s = 0; s < n_steps; s++) {
-    const long step = first_step + s;
+  // `count` steps from step0 with the walkers in G groups
+  auto run = [&](long step0, int count, int G) -> qmcb_status {
+  if (G > 1) {
+    qmcb_status gs = ensure_groups(ctx, G);
+    if (gs != QMCB_OK) return gs;
+    gs = groups_wait_main(ctx, G);
+    if (gs != QMCB_OK) return gs;
+  }
+  for (int s = 0; s < count; s++) {
+    const long step = step0 + s;
     const int m = (int)(step % ctx->N);
     const int sp = m < ctx->n[0] ? 0 : 1;
     // FP64 re-inversion of a spin's Slater matrices after refresh_interval
@@ -1466,6 +1469,98 @@ qmcb_status qmcb_vmc_steps(qmcb_ctx *ctx, int32_t num, int64_t first_step, int32
   if (G > 1) {
     qmcb_status gs = main_wait_groups(ctx, G);
     if (gs != QMCB_OK) return gs;
+  }
+  return QMCB_OK;
+  };
+  // walker groups: single determinant only; at least 256 walkers per group
+  auto clamp_groups = [&](int g) {
+    g = ctx->multi ? 1 : std::max(1, std::min<int>(g, qmcb_ctx::kMaxGroups));
+    while (g > 1 && num / g < 256) g--;
+    return g;
+  };
+  int done = 0;
+  int G = clamp_groups(ctx->opt.walker_groups);
+  if (ctx->opt.walker_groups < 0) {
+    // chosen once per context (again when the batch changes by more than 2x)
+    // by timing kTune steps of each candidate count on the real walkers, in
+    // this call and, if it is too short, the next ones (one group until the
+    // choice is made); the count changes no result bit, so the timed steps
+    // are ordinary steps.  A timed window holds no re-inversion: the steps up
+    // to one run untimed first.
+    const int kTune = ctx->opt.refresh_interval > 1 ? std::min(16, ctx->opt.refresh_interval - 1) : 16;
+    auto valid = [&]() {
+      return ctx->auto_groups > 0 && num <= 2 * ctx->auto_groups_num &&
+             2 * num >= ctx->auto_groups_num;
+    };
+    std::vector<int> cand;
+    for (int g = 1; g <= 3; g++) {
+      const int c = clamp_groups(g);
+      if (std::find(cand.begin(), cand.end(), c) == cand.end()) cand.push_back(c);
+    }
+    if (!valid() && cand.size() == 1) {
+      ctx->auto_groups = cand[0];
+      ctx->auto_groups_num = num;
+    } else if (!valid()) {
+      // one timed window per candidate, in this call or in the next ones
+      if (ctx->tune_num != num) {
+        ctx->tune_num = num;
+        ctx->tune_q = 0;
+      }
+      auto refresh_within = [&](long s0, int k) {
+        if (ctx->opt.refresh_interval <= 0) return -1;
+        long upd[2] = {ctx->sm_updates[0], ctx->sm_updates[1]};
+        for (int j = 0; j < k; j++) {
+          const int m = (int)((s0 + j) % ctx->N), sp = m < ctx->n[0] ? 0 : 1;
+          if (++upd[sp] >= ctx->opt.refresh_interval) return j;
+        }
+        return -1;
+      };
+      qmcb_status ts = ensure_groups(ctx, cand.back());
+      hipEvent_t e0 = nullptr, e1 = nullptr;
+      if (ts == QMCB_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+        ts = QMCB_DEVICE_ERROR;
+      while (ts == QMCB_OK && ctx->tune_q < (int)cand.size()) {
+        const int g = cand[ctx->tune_q];
+        bool room = true;
+        for (int j; ts == QMCB_OK && room && (j = refresh_within(first_step + done, kTune)) >= 0;) {
+          room = done + j + 1 + kTune <= n_steps;
+          if (room) {
+            ts = run(first_step + done, j + 1, g);
+            done += j + 1;
+          }
+        }
+        if (ts != QMCB_OK || !room || done + kTune > n_steps) break;
+        if (hipEventRecord(e0, ctx->stream) != hipSuccess) { ts = QMCB_DEVICE_ERROR; break; }
+        ts = run(first_step + done, kTune, g);
+        done += kTune;
+        float ms = 0.0f;
+        if (ts == QMCB_OK && (hipEventRecord(e1, ctx->stream) != hipSuccess ||
+                              hipEventSynchronize(e1) != hipSuccess ||
+                              hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+          ts = QMCB_DEVICE_ERROR;
+        if (ts != QMCB_OK) break;
+        ctx->tune_ms[ctx->tune_q++] = ms;
+      }
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+      if (ts != QMCB_OK) {
+        if (ts == QMCB_DEVICE_ERROR && ctx->err.empty()) ctx->err = "walker-group tuning failed";
+        return ts;
+      }
+      if (ctx->tune_q == (int)cand.size()) {  // every candidate timed: keep the fastest
+        int best = 0;
+        for (int q = 1; q < (int)cand.size(); q++)
+          if (ctx->tune_ms[q] < ctx->tune_ms[best]) best = q;
+        ctx->auto_groups = cand[best];
+        ctx->auto_groups_num = num;
+        ctx->tune_num = 0;
+      }
+    }
+    G = valid() ? clamp_groups(ctx->auto_groups) : 1;
+  }
+  {
+    qmcb_status rs = run(first_step + done, n_steps - done, G);
+    if (rs != QMCB_OK) return rs;
   }
   if (stats) {
     std::vector<double> a(num), b(num), c(num), d(num);
@@ -1585,9 +1680,10 @@ qmcb_status qmcb_dmc_steps(qmcb_ctx *ctx, int64_t first_step, int32_t n_steps,
   std::vector<double> wts(W), acc_e(W), acc_w(W);
   const CopyArgs cargs = walker_copy_args(ctx);
   long moves = 0;
-  // walker groups (single determinant): as in qmcb_vmc_steps, over the active list
-  const int G =
-      ctx->multi ? 1 : std::max(1, std::min<int>(ctx->opt.walker_groups, qmcb_ctx::kMaxGroups));
+  // walker groups (single determinant): as in qmcb_vmc_steps, over the active
+  // list; walker_groups < 0 uses the count a qmcb_vmc_steps call chose
+  const int want = ctx->opt.walker_groups >= 0 ? ctx->opt.walker_groups : ctx->auto_groups;
+  const int G = ctx->multi ? 1 : std::max(1, std::min<int>(want, qmcb_ctx::kMaxGroups));
   if (G > 1) {
     qmcb_status gs = ensure_groups(ctx, G);
     if (gs != QMCB_OK) return gs;
@@ -1825,8 +1921,14 @@ qmcb_status qmcb_sync(qmcb_ctx *ctx) {
 }
 
 qmcb_status qmcb_set_walker_groups(qmcb_ctx *ctx, int32_t groups) {
-  if (!ctx || groups < 0 || groups > qmcb_ctx::kMaxGroups) return QMCB_INVALID_ARG;
+  if (!ctx || groups < -1 || groups > qmcb_ctx::kMaxGroups) return QMCB_INVALID_ARG;
   ctx->opt.walker_groups = groups;
+  return QMCB_OK;
+}
+
+qmcb_status qmcb_get_walker_groups(const qmcb_ctx *ctx, int32_t *groups) {
+  if (!ctx || !groups) return QMCB_INVALID_ARG;
+  *groups = ctx->opt.walker_groups >= 0 ? ctx->opt.walker_groups : ctx->auto_groups;
   return QMCB_OK;
 }
 

@@ -144,3 +144,30 @@ def test_vmc_walker_groups_bit_identical(precision):
         np.testing.assert_array_equal(o[2], out[0][2])
         np.testing.assert_array_equal(o[3], out[0][3])
 
+
+def test_vmc_walker_groups_chosen_by_timing():
+    """walker_groups = -1: the first long call times windows of 1, 2 and 3
+    groups (without re-inversions) inside its own steps and keeps the
+    fastest; the trajectory, the call's sums and the per-sweep series equal
+    the one-group run bit for bit, and the next call keeps the choice."""
+    s = synth.make_system(1)
+    W, N, steps = 1024, s.n_elec, 200
+    R0 = synth.initial_walkers(s, W, 29)
+    out = []
+    for groups in (1, -1):
+        say("vmc walker groups", groups)
+        with Context(s.pack(), precision="fp32", max_walkers=W, seed=6, refresh_interval=64,
+                     walker_groups=groups) as ctx:
+            ctx.evaluate(np.arange(W), R0)
+            assert ctx.walker_groups() == (1 if groups == 1 else 0)
+            st = ctx.vmc_steps(W, 0, steps, 0.02)
+            chosen = ctx.walker_groups()
+            st2 = ctx.vmc_steps(W, steps, N, 0.02)
+            out.append((ctx.positions(W), (st.sum_e_local, st.sum_accept, st.sum_prop_e_local,
+                                           st2.sum_e_local, st2.sum_accept),
+                        ctx.sample_series(), chosen, ctx.walker_groups()))
+    say("chosen walker groups", out[1][3])
+    assert out[1][3] in (1, 2, 3) and out[1][4] == out[1][3]
+    np.testing.assert_array_equal(out[1][0], out[0][0])
+    assert out[1][1] == out[0][1]
+    np.testing.assert_array_equal(out[1][2], out[0][2])
