@@ -92,7 +92,8 @@ def run_path(s, precision, W, steps, warmup, tau, block, refresh, seed, groups=1
         k = min(block, steps - done)
         tb = time.perf_counter()
         st = ctx.vmc_steps(W, step, k, tau)
-        lat.append((time.perf_counter() - tb) * 1e3)
+        if k == block:  # latency statistics over full blocks only
+            lat.append((time.perf_counter() - tb) * 1e3)
         acc += st.sum_accept
         esum += st.sum_e_local
         step += k
@@ -165,13 +166,24 @@ def accuracy_report(s, n_conf=10000, seed=4321, tau=0.01, sweeps=1, precision="f
     return out
 
 
-def cpu_baseline(s, tau, seed, target_s=15.0):
+def cpu_model():
+    """The host CPU's model name (as lscpu prints it, from /proc/cpuinfo)."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(s, tau, seed, target_s=10.0):
     """The reference's own evaluate (oracle/_ref, compiled from its sources)
-    timed on this host's cores: one single-threaded reference process per
-    core (at most 16, the host share of one GPU), each on its own walkers,
-    all started together; the rate is the sum of their walker-moves over the
-    slowest one's time.  Without the reference build: the C oracle port on
-    one core."""
+    timed on this host's cores, two legs: one single-threaded reference
+    process alone (1 core), then one process per core (at most 16, the host
+    share of one GPU), each on its own walkers, all started together, the
+    rate being the sum of their walker-moves over the slowest one's time.
+    Without the reference build: the C oracle port on one core."""
     from qmcbeaver_amd import synth
     drv = os.path.join(ROOT, "oracle", "_ref", "qmcb_ref_driver")
     nw = 8
@@ -196,6 +208,12 @@ def cpu_baseline(s, tau, seed, target_s=15.0):
             r = json.loads(out.strip().splitlines()[-1])
             rate = r["walker_moves"] / max(r["seconds"], 1e-9)
             moves = int(max(16, min(4096, target_s * rate / nw)))
+            print(f"cpu baseline: 1 core, {nw} walkers x {moves} moves", file=sys.stderr,
+                  flush=True)
+            out = subprocess.run([drv, "vmc", ck, cfs[0], str(moves), str(tau), "7"], cwd=tmp,
+                                 capture_output=True, text=True, check=True).stdout
+            one = json.loads(out.strip().splitlines()[-1])
+            print(f"cpu baseline: {cores} processes", file=sys.stderr, flush=True)
             procs = [subprocess.Popen([drv, "vmc", ck, cf, str(moves), str(tau), str(7 + c)],
                                       cwd=tmp, stdout=subprocess.PIPE, stderr=subprocess.DEVNULL,
                                       text=True) for c, cf in enumerate(cfs)]
@@ -208,8 +226,10 @@ def cpu_baseline(s, tau, seed, target_s=15.0):
         total = sum(r["walker_moves"] for r in res)
         slowest = max(r["seconds"] for r in res)
         return {"value": total / slowest, "unit": "walker-moves/s", "cores": cores,
-                "kind": "reference",
-                "per_core": res[0]["walker_moves"] / res[0]["seconds"],
+                "kind": "reference", "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+                "one_core": {"value": one["walker_moves"] / one["seconds"], "cores": 1,
+                             "sample": f"1 process x {nw} walkers x {moves} single-electron "
+                                       f"VMC moves, {one['seconds']:.1f} s"},
                 "sample": f"{cores} processes x {nw} walkers x {moves} single-electron VMC moves "
                           f"of the same molecule (reference QMCSCFJastrow::evaluate, "
                           f"re-evaluate on rejection), slowest {slowest:.1f} s"}
@@ -221,6 +241,7 @@ def cpu_baseline(s, tau, seed, target_s=15.0):
     moves = int(max(16, min(4096, target_s * rate / nw)))
     t, _ = o.vmc_time(R, moves, tau)
     return {"value": nw * moves / t, "unit": "walker-moves/s", "cores": 1, "kind": "port",
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(),
             "sample": f"{nw} walkers x {moves} single-electron VMC moves (C oracle), {t:.1f} s"}
 
 
@@ -398,7 +419,12 @@ def main():
     avg_ms = kt["ms"] / max(kt["launches"], 1)
     bytes_launch = args.walkers * sm_bytes_per_move(n, N, s_d, s_d, mixed["acceptance"])
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
-    lat = np.array(mixed["block_ms"])
+    def block_stats(ms):
+        """p50/p99 over the full blocks of the timed run (None without one)."""
+        if not ms:
+            return None
+        return {"p50": float(np.percentile(ms, 50)), "p99": float(np.percentile(ms, 99)),
+                "blocks": len(ms), "block_steps": args.block}
     out = {
         "metric": METRIC,
         "value": mixed["walker_moves_per_s"],
@@ -425,13 +451,11 @@ def main():
         "fp64": None if fp64 is None else {
             "value": fp64["walker_moves_per_s"], "ms_per_step": fp64["ms_per_step"],
             "acceptance": fp64["acceptance"], "mean_e_local": fp64["mean_e_local"],
-            "block_latency_ms": {"p50": float(np.percentile(fp64["block_ms"], 50)),
-                                 "p99": float(np.percentile(fp64["block_ms"], 99))}},
+            "mol256_block_latency_ms": block_stats(fp64["block_ms"])},
         "mixed_over_fp64": None if fp64 is None else
         mixed["walker_moves_per_s"] / fp64["walker_moves_per_s"],
-        "block_latency_ms": {"p50": float(np.percentile(lat, 50)),
-                             "p99": float(np.percentile(lat, 99)),
-                             "block_steps": args.block},
+        "block_latency_ms": block_stats(mixed["block_ms"]),
+        "mol256_block_latency_ms": block_stats(mixed["block_ms"]),
         "acceptance": mixed["acceptance"],
         "mean_e_local": mixed["mean_e_local"],
         "ms_per_step_one_group_with_kernel_events": mixed["ms_per_step_one_group_timed"],
@@ -494,6 +518,11 @@ def main():
                 "block_latency_ms": srv64["block_latency_ms"],
                 "walker_moves_per_s": srv64["walker_moves_per_s"], "wall_s": srv64["wall_s"],
                 "completed": srv64["completed"], "failed": srv64["failed"]}}
+        # the metric's p50/p99 block latency is the serving loop's (blocks of
+        # 100 steps of jobs arriving while others run); the mol256 run's
+        # blocks are reported beside it
+        out["block_latency_ms"] = dict(srv["block_latency_ms"], block_steps=100,
+                                       source="serving loop, mixed precision")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(s, args.tau, 99)
     print(json.dumps(out))

@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define QMCB_ABI_VERSION 1
+#define QMCB_ABI_VERSION 2
 #define QMCB_PADE_MAX 8
 
 typedef enum qmcb_status {
@@ -105,10 +105,14 @@ typedef struct qmcb_options {
   int32_t precision;        /* qmcb_precision                                  */
   int32_t max_walkers;      /* resident walker slots                           */
   int32_t drift_mod;        /* qmcb_drift_mod                                  */
-  int32_t refresh_interval; /* device VMC: FP64 re-inversion of a spin's Slater
-                               matrices after K single-electron moves of that
-                               spin; 0 = never (the reference never re-inverts
-                               in single-electron mode)                        */
+  int32_t refresh_interval; /* device VMC/DMC: each walker's Slater matrices of
+                               a spin are refreshed to FP64 accuracy once every
+                               K single-electron moves of that spin, staggered
+                               by slot (slot s at the moves c of its spin with
+                               c mod K = s mod K), so every step refreshes
+                               about 1/K of the walkers of the moved spin; 0 =
+                               never (the reference never re-inverts in
+                               single-electron mode)                          */
   double dt;                /* time step used by the drift modification        */
   double umrigar_a;         /* umrigar93_equalelectrons_parameter              */
   uint64_t seed;            /* device RNG seed (vmc entry points)              */
@@ -132,6 +136,13 @@ typedef struct qmcb_options {
                                then) and keeps the fastest, again when the
                                batch changes by more than 2x; qmcb_dmc_steps
                                uses that choice                              */
+  int32_t old_walker_acceptance; /* old_walker_acceptance_parameter of the
+                               reference (QMCFlags.cpp:59, default 50): after
+                               more than this many consecutive rejected moves a
+                               walker's acceptance probability is multiplied by
+                               1.1^(age - K) (QMCWalker.cpp:1578-1600) and, in
+                               DMC, its weight decays by 0.9 per sweep instead
+                               of being reweighted (:1181-1184).  <= 0: off    */
 } qmcb_options;
 
 /* Caller-owned outputs, one row per evaluated walker.  Any pointer may be
@@ -252,6 +263,25 @@ typedef struct qmcb_dmc_params {
                                |modified drift| / |drift| (QMCWalker.cpp:1121-
                                1132); needs qmcb_options.drift_mod
                                QMCB_DRIFT_UMRIGAR93_UNEQUAL to differ from 0   */
+  double rel_cutoff;        /* the reference's rel_cutoff (QMCFlags.cpp:61,
+                               default 100): at a reweighting, a walker with
+                               |E_L - E_0| / |E_0| > rel_cutoff is deleted
+                               (weight 0, QMCWalker.cpp:1270-1290, 1386-1393);
+                               E_0 = the e_trial given at the first call after
+                               qmcb_dmc_init.  <= 0: no such deletion          */
+  int32_t equilibration_steps; /* steps [0, equilibration_steps) are the
+                               reference's equilibration (iteration < 0):
+                               E_T = E_est - ln(W_total / target) / tau
+                               (QMCManager.cpp:2277-2291), E_est stays E_0, and
+                               while more than 100 of them remain the stricter
+                               deletion rules hold (heavy, light, dW outside
+                               [0.5, 1.5], age >= 30; QMCWalker.cpp:1207-1300).
+                               Production (the rest): a walker is deleted when
+                               dW^(age+1) > 15, its weight exceeds 50 or its
+                               energy is bad (:1312-1393), and E_est is the
+                               weighted mean of the production sweeps' samples
+                               once there are at least 100 of them
+                               (QMCManager.cpp:2247-2261), E_0 before          */
 } qmcb_dmc_params;
 
 typedef struct qmcb_dmc_stats {
@@ -264,6 +294,13 @@ typedef struct qmcb_dmc_stats {
   int32_t n_split, n_killed, n_fused;
   double total_weight;
   double e_trial;           /* trial energy after the last update              */
+  int32_t n_split_skipped;  /* walkers above branch_threshold left unsplit
+                               because every slot was in use (they keep their
+                               weight; the reference always splits,
+                               QMCRun.cpp:748-757): raise max_walkers          */
+  int32_t n_deleted;        /* walkers removed by the deletion rules (weight
+                               set to 0 at a reweighting; part of n_killed)    */
+  double e_estimated;       /* E_est after the last update                     */
 } qmcb_dmc_stats;
 
 /* Slots [0, num) (all-electron evaluated) become the DMC population, weight 1. */
@@ -280,8 +317,10 @@ qmcb_status qmcb_sync(qmcb_ctx *ctx);
  * at the next qmcb_vmc_steps.  No reference counterpart (a scheduling knob). */
 qmcb_status qmcb_set_walker_groups(qmcb_ctx *ctx, int32_t groups);
 
-/* The walker-group count in use: the option, or with walker_groups = -1 the
- * count chosen by timing (0 while none is chosen yet).  No reference
+/* The walker-group count the last qmcb_vmc_steps / qmcb_dmc_steps call ran
+ * its steps with (after the clamps to at least 256 walkers per group and one
+ * group with several determinants; while a walker_groups = -1 choice is being
+ * timed, the count of its last step); 0 before any such call.  No reference
  * counterpart (a scheduling knob). */
 qmcb_status qmcb_get_walker_groups(const qmcb_ctx *ctx, int32_t *groups);
 

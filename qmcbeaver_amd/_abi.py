@@ -44,7 +44,8 @@ class Options(C.Structure):
                 ("drift_mod", C.c_int32), ("refresh_interval", C.c_int32),
                 ("dt", C.c_double), ("umrigar_a", C.c_double),
                 ("seed", C.c_uint64), ("device", C.c_int32), ("replace_cusps", C.c_int32),
-                ("basis_scratch_bytes", C.c_int64), ("walker_groups", C.c_int32)]
+                ("basis_scratch_bytes", C.c_int64), ("walker_groups", C.c_int32),
+                ("old_walker_acceptance", C.c_int32)]
 
 
 class Results(C.Structure):
@@ -70,7 +71,8 @@ class DmcParams(C.Structure):
     _fields_ = [("tau", C.c_double), ("e_trial", C.c_double), ("pop_control", C.c_double),
                 ("branch_threshold", C.c_double), ("fusion_threshold", C.c_double),
                 ("target_walkers", C.c_int32), ("reweight_after", C.c_int32),
-                ("energy_modification", C.c_int32)]
+                ("energy_modification", C.c_int32), ("rel_cutoff", C.c_double),
+                ("equilibration_steps", C.c_int32)]
 
 
 class DmcStats(C.Structure):
@@ -78,7 +80,8 @@ class DmcStats(C.Structure):
                 ("sum_weights", C.c_double), ("sum_accept", C.c_double),
                 ("n_moves", C.c_int64), ("population", C.c_int32), ("n_split", C.c_int32),
                 ("n_killed", C.c_int32), ("n_fused", C.c_int32), ("total_weight", C.c_double),
-                ("e_trial", C.c_double)]
+                ("e_trial", C.c_double), ("n_split_skipped", C.c_int32),
+                ("n_deleted", C.c_int32), ("e_estimated", C.c_double)]
 
 
 class PropertySummary(C.Structure):

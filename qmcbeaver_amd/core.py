@@ -131,7 +131,7 @@ def cusp_fit(packed: Dict, spin: int = 0) -> np.ndarray:
 
 def _options(precision, max_walkers, drift_mod=QMCB_DRIFT_NONE, dt=0.1, umrigar_a=0.5,
              refresh_interval=0, seed=12345, device=0, replace_cusps=False,
-             basis_scratch_bytes=0, walker_groups=0) -> Options:
+             basis_scratch_bytes=0, walker_groups=0, old_walker_acceptance=50) -> Options:
     o = Options()
     o.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
     o.max_walkers = max_walkers
@@ -144,6 +144,7 @@ def _options(precision, max_walkers, drift_mod=QMCB_DRIFT_NONE, dt=0.1, umrigar_
     o.replace_cusps = 1 if replace_cusps else 0
     o.basis_scratch_bytes = int(basis_scratch_bytes)
     o.walker_groups = int(walker_groups)
+    o.old_walker_acceptance = int(old_walker_acceptance)
     return o
 
 
@@ -182,13 +183,17 @@ class Context:
                  drift_mod: int = QMCB_DRIFT_NONE, dt: float = 0.1, umrigar_a: float = 0.5,
                  refresh_interval: int = 0, seed: int = 12345, device: int = 0,
                  replace_cusps: bool = False, basis_scratch_bytes: int = 0,
-                 walker_groups: int = 0):
+                 walker_groups: int = 0, old_walker_acceptance: int = 50):
+        """old_walker_acceptance: the reference's old_walker_acceptance_parameter
+        (acceptance boost and DMC weight decay of walkers stuck for more than
+        this many moves); 0 switches the rule off."""
         self._buf = SystemBuffers(packed)
         self.N = self._buf.N
         self.n_up = int(packed["n_up"])
         self.max_walkers = max_walkers
         o = _options(precision, max_walkers, drift_mod, dt, umrigar_a, refresh_interval, seed,
-                     device, replace_cusps, basis_scratch_bytes, walker_groups)
+                     device, replace_cusps, basis_scratch_bytes, walker_groups,
+                     old_walker_acceptance)
         self._opt = o
         h = C.c_void_p()
         st = lib().qmcb_create(C.byref(self._buf.struct), C.byref(o), C.byref(h))
@@ -254,11 +259,17 @@ class Context:
     def dmc_steps(self, first_step: int, n_steps: int, tau: float, e_trial: float, *,
                   target: int = 0, pop_control: float = 1.0, branch_threshold: float = 2.0,
                   fusion_threshold: float = 0.45, reweight_after: int = 100,
-                  energy_modification: bool = False) -> DmcStats:
+                  energy_modification: bool = False, rel_cutoff: float = 100.0,
+                  equilibration_steps: int = 0) -> DmcStats:
         """energy_modification: the reference's umrigar93 energy modification
-        of the weights (meaningful with drift_mod=QMCB_DRIFT_UMRIGAR93_UNEQUAL)."""
+        of the weights (meaningful with drift_mod=QMCB_DRIFT_UMRIGAR93_UNEQUAL).
+        rel_cutoff: delete walkers whose energy is further than rel_cutoff x
+        |E_0| from E_0 (the first e_trial; 0: off).  equilibration_steps: the
+        reference's equilibration phase (its trial-energy update and
+        stricter deletion rules)."""
         p = DmcParams(tau, e_trial, pop_control, branch_threshold, fusion_threshold, target,
-                      reweight_after, 1 if energy_modification else 0)
+                      reweight_after, 1 if energy_modification else 0, rel_cutoff,
+                      equilibration_steps)
         st = DmcStats()
         self._check(lib().qmcb_dmc_steps(self._h, first_step, n_steps, C.byref(p),
                                          C.byref(st)), "qmcb_dmc_steps")

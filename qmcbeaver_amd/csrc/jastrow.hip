@@ -64,26 +64,26 @@ __global__ void __launch_bounds__(kJT) jastrow_all_kernel(JastrowArgs A, const i
     for (int A_ = 0; A_ < nn; A_++) {
       double d[3], r2 = 0.0;
       for (int k = 0; k < 3; k++) { d[k] = x[3 * i + k] - S.nuc_xyz[3 * A_ + k]; r2 += d[k] * d[k]; }
-      const double r = sqrt(r2);
+      const double r = sqrt(r2), ir = 1.0 / r;
       double u, du, d2u;
       pade_eval(S.en[sp][A_], r, u, du, d2u);
       U += u;
-      Uxx += 2.0 / r * du + d2u;
-      for (int k = 0; k < 3; k++) g[k] += du * (d[k] / r);
-      Ven -= S.nuc_z[A_] / r;
+      Uxx += 2.0 * ir * du + d2u;
+      for (int k = 0; k < 3; k++) g[k] += du * ir * d[k];
+      Ven -= S.nuc_z[A_] * ir;
     }
     for (int j = 0; j < N; j++) {
       if (j == i) continue;
       double d[3], r2 = 0.0;
       for (int k = 0; k < 3; k++) { d[k] = x[3 * i + k] - x[3 * j + k]; r2 += d[k] * d[k]; }
-      const double r = sqrt(r2);
+      const double r = sqrt(r2), ir = 1.0 / r;
       double u, du, d2u;
       pade_eval(S.ee[ee_type(i, j, nu)], r, u, du, d2u);
-      for (int k = 0; k < 3; k++) g[k] += du * (d[k] / r);
+      for (int k = 0; k < 3; k++) g[k] += du * ir * d[k];
       if (j < i) {
         U += u;
-        Uxx += 2.0 * (2.0 / r * du + d2u);
-        Vee += 1.0 / r;
+        Uxx += 2.0 * (2.0 * ir * du + d2u);
+        Vee += ir;
       }
     }
     double *ux = A.Ux + ((long)w * N + i) * 3;
@@ -142,18 +142,19 @@ jastrow_move_kernel(JastrowArgs A, const double *cU, const double *cUx, const do
       dn[k] = xn[k] - xj[k]; rn2 += dn[k] * dn[k];
       dold[k] = xo[k] - xj[k]; ro2 += dold[k] * dold[k];
     }
-    const double rn = sqrt(rn2), ro = sqrt(ro2);
+    const double rn = sqrt(rn2), ro = sqrt(ro2), irn = 1.0 / rn, iro = 1.0 / ro;
     const qmcb_pade &f = S.ee[ee_type(m, j, nu)];
     double un, dun, d2un, uo, duo, d2uo;
     pade_eval(f, rn, un, dun, d2un);
     pade_eval(f, ro, uo, duo, d2uo);
     dU += un - uo;
-    dUxx += 2.0 * (2.0 / rn * dun + d2un) - 2.0 * (2.0 / ro * duo + d2uo);
-    dVee += 1.0 / rn - 1.0 / ro;
+    dUxx += 2.0 * (2.0 * irn * dun + d2un) - 2.0 * (2.0 * iro * duo + d2uo);
+    dVee += irn - iro;
+    const double gn = dun * irn, go = duo * iro;
     for (int k = 0; k < 3; k++) {
       // gradient on j: u'(r) (x_j - x_m)/r
-      pux[3 * j + k] = ux[3 * j + k] - dun * (dn[k] / rn) + duo * (dold[k] / ro);
-      gm[k] += dun * (dn[k] / rn);
+      pux[3 * j + k] = ux[3 * j + k] - gn * dn[k] + go * dold[k];
+      gm[k] += gn * dn[k];
     }
   }
   const int sp = m < nu ? 0 : 1;
@@ -163,14 +164,15 @@ jastrow_move_kernel(JastrowArgs A, const double *cU, const double *cUx, const do
       dn[k] = xn[k] - S.nuc_xyz[3 * a_ + k]; rn2 += dn[k] * dn[k];
       dold[k] = xo[k] - S.nuc_xyz[3 * a_ + k]; ro2 += dold[k] * dold[k];
     }
-    const double rn = sqrt(rn2), ro = sqrt(ro2);
+    const double rn = sqrt(rn2), ro = sqrt(ro2), irn = 1.0 / rn, iro = 1.0 / ro;
     double un, dun, d2un, uo, duo, d2uo;
     pade_eval(S.en[sp][a_], rn, un, dun, d2un);
     pade_eval(S.en[sp][a_], ro, uo, duo, d2uo);
     dU += un - uo;
-    dUxx += (2.0 / rn * dun + d2un) - (2.0 / ro * duo + d2uo);
-    dVen += -S.nuc_z[a_] / rn + S.nuc_z[a_] / ro;
-    for (int k = 0; k < 3; k++) gm[k] += dun * (dn[k] / rn);
+    dUxx += (2.0 * irn * dun + d2un) - (2.0 * iro * duo + d2uo);
+    dVen += S.nuc_z[a_] * (iro - irn);
+    const double gn = dun * irn;
+    for (int k = 0; k < 3; k++) gm[k] += gn * dn[k];
   }
   double v[7] = {dU, dUxx, dVee, dVen, gm[0], gm[1], gm[2]};
   block_sum<kMT>(v, 7, scratch);
@@ -272,6 +274,7 @@ __global__ void __launch_bounds__(kJT) assemble_kernel(AssembleArgs A, const int
     if (o.jastrow_u) o.jastrow_u[b] = A.U[w];
     if (o.singular) o.singular[b] = sing ? 1 : 0;
     if (A.cur_el) A.cur_el[w] = EL;
+    if (A.age) A.age[w] = 0;
   }
 }
 

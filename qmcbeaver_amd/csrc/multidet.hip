@@ -356,6 +356,7 @@ mdet_assemble_kernel(MdetArgs A, const int *walkers, int num) {
   if (staged) A.st_row_singular[b] = sing ? 1 : 0;
   if (!staged) {
     if (A.cur_el) A.cur_el[w0] = EL;
+    if (A.walk.age) A.walk.age[w0] = 0;
     return;
   }
   if (A.mode < 1) return;
@@ -367,10 +368,15 @@ mdet_assemble_kernel(MdetArgs A, const int *walkers, int num) {
   double p = (sing || s_sc[5] != 0.0)
                  ? 0.0
                  : exp(2.0 * (logpsi - logpsi_old) - (s_red[3] - s_red[2]) / (2.0 * tau));
+  const bool bad = sing || s_sc[5] != 0.0;
+  const int age_old = A.walk.age ? A.walk.age[w0] : 0;
   if (!(p == p)) p = 0.0;
+  else if (!bad) p = old_walker_boost(p, age_old, A.walk.old_walker);
   if (A.mode == 2 && s_sc[1] * s_sc[4] < 0.0) p = 0.0;
   if (p > 1.0) p = 1.0;
   const bool accept = p > A.unif[b];
+  const int age_new = accept ? 0 : age_old + 1;  // QMCWalker.cpp:1656-1672
+  if (A.walk.age) A.walk.age[w0] = age_new;
   const double el_old = A.cur_el[w0];
   A.mask[b] = accept ? 1 : 0;
   if (accept) A.cur_el[w0] = EL;
@@ -388,7 +394,8 @@ mdet_assemble_kernel(MdetArgs A, const int *walkers, int num) {
         st_ = A.e_trial - EL;
       }
       if (!(p > 0.0)) st_ = 0.0;
-      wt *= exp(A.tau_eff * (p * 0.5 * (so_ + st_) + (1.0 - p) * so_));
+      const double dW = exp(A.tau_eff * (p * 0.5 * (so_ + st_) + (1.0 - p) * so_));
+      wt = reweight_walker(wt, dW, age_new, el, A.walk);
       A.weight[w0] = wt;
     }
     A.acc_w[w0] += wt;

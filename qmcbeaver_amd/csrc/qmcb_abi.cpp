@@ -59,10 +59,9 @@ struct qmcb_ctx {
   size_t dev_bytes = 0;  // device memory this context holds (arena + own allocations)
   char *arena = nullptr;
   size_t arena_cap = 0, arena_used = 0;
-  // single-electron orbital GEMM: waves per workgroup chosen by tune_gemm
-  // (0 = not tuned yet) and the batch size it was tuned at, per spin
-  int gemm_wpb[2] = {0, 0};
-  int gemm_tuned_num[2] = {0, 0};
+  // single-electron orbital GEMM: waves per workgroup chosen by tune_gemm,
+  // per (spin, batch rows / 16)
+  std::map<std::pair<int, int>, int> gemm_wpb;
   size_t ts = 8, tx = 8, to = 8;  // bytes: derivative storage, GEMM operand, GEMM output
 
   // resident walker state
@@ -77,7 +76,14 @@ struct qmcb_ctx {
   double *ubuf[2] = {nullptr, nullptr}, *zbuf[2] = {nullptr, nullptr};
   double *U = nullptr, *Ux = nullptr, *Uxx = nullptr, *Vee = nullptr, *Ven = nullptr;
   double *cur_el = nullptr;
-  // batch-row staging
+  int *age = nullptr;  // [Wc] consecutive rejected moves (QMCWalker::age)
+  // staggered refresh: the walker list sorted by slot residue mod K (see
+  // refresh_plan), valid for the VMC identity list of rplan_num walkers, or
+  // built from the DMC active list (rplan_num = -1)
+  int *d_rmap = nullptr;
+  std::vector<int> rmap, rpos, roff;
+  int rplan_num = 0, rplan_K = 0;
+  long moves_of_spin[2] = {0, 0};  // single-electron moves of each spin so far
   double *Rin = nullptr;
   double *propR = nullptr, *gauss = nullptr, *unif = nullptr;
   double *pU = nullptr, *pUx = nullptr, *pUxx = nullptr, *pVee = nullptr, *pVen = nullptr;
@@ -122,17 +128,21 @@ struct qmcb_ctx {
   bool have_e_trial = false;
   double e_trial = 0.0;
   double e_est = 0.0;  // weighted mean local energy the trial energy is fed back from
+  double e_orig = 0.0;  // energy_estimated_original: the first e_trial after dmc_init
+  // production samples (one per sweep) the estimate averages, across calls
+  double est_we = 0.0, est_w = 0.0;
+  long est_n = 0;
   // staged single-electron pass
   int staged_moved = -1;
   int staged_num = 0;
   std::vector<int> staged_ids;
   std::map<int, int> staged_row;
   std::vector<int> initialized;  // per slot: has a resident state
-  long sm_updates[2] = {0, 0};   // SM updates per spin since the last re-inversion
   int auto_groups = 0;           // walker_groups < 0: the count chosen by timing (0: not yet)
   int auto_groups_num = 0;       // ... for this many walkers
-  int tune_num = 0, tune_q = 0;  // timing in progress: batch size, candidates timed so far
-  float tune_ms[3] = {0, 0, 0};
+  int tune_num = 0, tune_q = 0;  // timing in progress: batch size, windows run so far
+  float tune_ms[3] = {0, 0, 0};  // best window per candidate
+  int last_groups = 0;           // groups the last vmc/dmc steps ran with
   // timing
   bool timing = false;
   Timing tm;
@@ -323,23 +333,25 @@ MdetArgs mdet_args(qmcb_ctx *ctx, int m, int mode, const qmcb_results &out) {
   a.st_cgrad = ctx->st_cgrad;
   a.weight = ctx->weight;
   a.acc_w = ctx->acc_w;
+  a.walk.age = ctx->age;
+  a.walk.old_walker = ctx->opt.old_walker_acceptance;
   return a;
 }
 
 // ---- tile autotuning -----------------------------------------------------------
 // The single-electron orbital GEMM has one row per walker, so its grid is
 // small (num / 64 workgroups per component) and how evenly it spreads over
-// the CUs decides its speed.  The first move of a spin (and any later call
-// whose batch differs by more than 2x from the tuned one) times the 4-wave
-// and 2-wave workgroup variants on the real operands, three launches each,
-// and keeps the faster; batches below 4096 rows (where the timing runs would
-// cost more than they can save) take the 4-wave variant.  Both variants
-// produce identical bits.
+// the CUs decides its speed.  The first move of a spin at a batch size (per
+// 16 rows; walker groups run at their own sizes) times the 4-wave and 2-wave
+// workgroup variants on the real operands, three launches each, and keeps
+// the faster; batches below 4096 rows (where the timing runs would cost more
+// than they can save) take the 4-wave variant.  Both variants produce
+// identical bits.
 template <typename F>
 qmcb_status tune_gemm(qmcb_ctx *ctx, int sp, int num, F &&gemm) {
+  const std::pair<int, int> key(sp, num >> 4);
   if (num < 4096) {  // small batches: the 4-wave variant, no timing runs
-    ctx->gemm_wpb[sp] = 4;
-    ctx->gemm_tuned_num[sp] = num;
+    ctx->gemm_wpb[key] = 4;
     return QMCB_OK;
   }
   hipEvent_t e0, e1;
@@ -359,8 +371,7 @@ qmcb_status tune_gemm(qmcb_ctx *ctx, int sp, int num, F &&gemm) {
     }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  ctx->gemm_wpb[sp] = best_ms[1] < best_ms[0] ? cand[1] : cand[0];
-  ctx->gemm_tuned_num[sp] = num;
+  ctx->gemm_wpb[key] = best_ms[1] < best_ms[0] ? cand[1] : cand[0];
   return QMCB_OK;
 }
 
@@ -424,6 +435,7 @@ struct Engine {
     aa.umrigar_a = ctx->opt.umrigar_a;
     aa.out = out;
     aa.cur_el = ctx->cur_el;
+    aa.age = ctx->age;
     TIMED("assemble", launch_assemble(aa, d_map, num, ctx->stream));
     return QMCB_OK;
   }
@@ -450,12 +462,13 @@ struct Engine {
                                          (const TX *)ctx->X, ctx->kpad, ctx->ldp, (long)num,
                                          ctx->kpad, go, ctx->stream, wpb);
     };
-    if (ctx->gemm_wpb[sp] == 0 || num > 2 * ctx->gemm_tuned_num[sp] ||
-        2 * num < ctx->gemm_tuned_num[sp]) {
+    auto it = ctx->gemm_wpb.find(std::make_pair(sp, num >> 4));
+    if (it == ctx->gemm_wpb.end()) {
       qmcb_status ts = tune_gemm(ctx, sp, num, gemm);
       if (ts != QMCB_OK) return ts;
+      it = ctx->gemm_wpb.find(std::make_pair(sp, num >> 4));
     }
-    TIMED("orbital_gemm", gemm(ctx->gemm_wpb[sp]));
+    TIMED("orbital_gemm", gemm(it->second));
     (void)n;
     if (ctx->cusp) TIMED("cusp", (launch_cusp<TO>(ctx->cuspd[sp], pts, go, ctx->stream)));
     return QMCB_OK;
@@ -492,6 +505,8 @@ struct Engine {
     a.acc_prop_el = ctx->acc_prop_el; a.cur_el = ctx->cur_el;
     a.weight = ctx->weight;
     a.acc_w = ctx->acc_w;
+    a.walk.age = ctx->age;
+    a.walk.old_walker = ctx->opt.old_walker_acceptance;
     return a;
   }
 
@@ -827,6 +842,8 @@ qmcb_status create_impl(const qmcb_system *s, const qmcb_options *o, qmcb_ctx *c
   QCHK(dalloc(ctx, &ctx->Vee, W));
   QCHK(dalloc(ctx, &ctx->Ven, W));
   QCHK(dalloc(ctx, &ctx->cur_el, W));
+  QCHK(dalloc(ctx, &ctx->age, W));
+  QCHK(dalloc(ctx, &ctx->d_rmap, W));
   QCHK(dalloc(ctx, &ctx->Rin, W * N * 3));
   QCHK(dalloc(ctx, &ctx->propR, W * 3));
   QCHK(dalloc(ctx, &ctx->gauss, W * 3));
@@ -1010,6 +1027,8 @@ size_t plan_device_bytes(const qmcb_system *s, const qmcb_options *o) {
   add(W * N * 3, 8);
   if (multi) add(W * N * 3, 8);
   for (int i = 0; i < 4; i++) add(W, 8);   // Uxx, Vee, Ven, cur_el
+  add(W, 4);                                 // age
+  add(W, 4);                                 // refresh map
   add(W * N * 3, 8);                         // Rin
   add(W * 3, 8);
   add(W * 3, 8);
@@ -1342,6 +1361,50 @@ static qmcb_status series_reserve(qmcb_ctx *ctx, int64_t first_step, int32_t n_s
   return QMCB_OK;
 }
 
+// Staggered refresh.  Slot s of a spin is refreshed (FP64 re-inversion and
+// traces) right after the single-electron moves c of that spin (counted over
+// the context's life) with c mod K = s mod K, on the stream of the walker
+// group that moved it: every step re-inverts about 1/K of the walkers of the
+// moved spin, so any window of steps carries its share of the refresh work.
+// The plan sorts the walker list the steps run over (slots = null: the
+// identity list of the VMC; else the DMC active list) by residue, then list
+// position: rmap[roff[j] .. roff[j + 1]) are the slots with s mod K = j and
+// rpos their list positions, so the share of a walker group (a contiguous
+// range of list positions) is a contiguous piece of rmap.  rmap is copied to
+// d_rmap; the caller synchronises the stream before rmap changes again.
+static qmcb_status refresh_plan(qmcb_ctx *ctx, const int *slots, int num) {
+  const int K = ctx->opt.refresh_interval;
+  ctx->rmap.assign(std::max(num, 1), 0);
+  ctx->rpos.assign(std::max(num, 1), 0);
+  ctx->roff.assign((size_t)K + 1, 0);
+  for (int i = 0; i < num; i++) ctx->roff[(slots ? slots[i] : i) % K + 1]++;
+  for (int j = 0; j < K; j++) ctx->roff[j + 1] += ctx->roff[j];
+  std::vector<int> fill(ctx->roff.begin(), ctx->roff.end() - 1);
+  for (int i = 0; i < num; i++) {
+    const int sl = slots ? slots[i] : i, q = fill[sl % K]++;
+    ctx->rmap[q] = sl;
+    ctx->rpos[q] = i;
+  }
+  if (num > 0)
+    QCHK(hipMemcpyAsync(ctx->d_rmap, ctx->rmap.data(), sizeof(int) * num, hipMemcpyHostToDevice,
+                        ctx->stream));
+  ctx->rplan_num = slots ? -1 : num;
+  ctx->rplan_K = K;
+  return QMCB_OK;
+}
+
+// The refresh share, after a move of spin sp, of the walkers at list
+// positions [first, last).
+static qmcb_status staggered_refresh(qmcb_ctx *ctx, long first, long last, int sp) {
+  const int K = ctx->rplan_K;
+  if (K <= 0) return QMCB_OK;
+  const int j = (int)(ctx->moves_of_spin[sp] % K);
+  const int *p0 = ctx->rpos.data() + ctx->roff[j], *p1 = ctx->rpos.data() + ctx->roff[j + 1];
+  const int *b = std::lower_bound(p0, p1, (int)first), *e = std::lower_bound(b, p1, (int)last);
+  if (e == b) return QMCB_OK;
+  return DISPATCH(ctx, refresh(ctx, ctx->d_rmap + (b - ctx->rpos.data()), (int)(e - b), 1 << sp));
+}
+
 qmcb_status qmcb_vmc_steps(qmcb_ctx *ctx, int32_t num, int64_t first_step, int32_t n_steps,
                            double tau, qmcb_vmc_stats *stats) {
   if (!ctx || num < 1 || num > ctx->Wc || n_steps < 0 || first_step < 0 || !(tau > 0))
@@ -1365,6 +1428,7 @@ qmcb_status qmcb_vmc_steps(qmcb_ctx *ctx, int32_t num, int64_t first_step, int32
       if (rs != QMCB_OK) return rs;
     }
   }
+  const int K = ctx->opt.refresh_interval;
   qmcb_results none{};
   auto propose = [&](long step, int m, const int *d_map, int cnt) -> qmcb_status {
     ProposeArgs pa{};
@@ -1388,89 +1452,88 @@ qmcb_status qmcb_vmc_steps(qmcb_ctx *ctx, int32_t num, int64_t first_step, int32
   };
   // `count` steps from step0 with the walkers in G groups
   auto run = [&](long step0, int count, int G) -> qmcb_status {
-  if (G > 1) {
-    qmcb_status gs = ensure_groups(ctx, G);
-    if (gs != QMCB_OK) return gs;
-    gs = groups_wait_main(ctx, G);
-    if (gs != QMCB_OK) return gs;
-  }
-  for (int s = 0; s < count; s++) {
-    const long step = step0 + s;
-    const int m = (int)(step % ctx->N);
-    const int sp = m < ctx->n[0] ? 0 : 1;
-    // FP64 re-inversion of a spin's Slater matrices after refresh_interval
-    // Sherman-Morrison updates of that spin (all walkers move in lockstep)
-    bool do_refresh = false;
-    if (ctx->opt.refresh_interval > 0 && ++ctx->sm_updates[sp] >= ctx->opt.refresh_interval) {
-      ctx->sm_updates[sp] = 0;
-      do_refresh = true;
+    if (K > 0 && !(ctx->rplan_num == num && ctx->rplan_K == K)) {
+      qmcb_status ps = refresh_plan(ctx, nullptr, num);
+      if (ps != QMCB_OK) return ps;
+      QCHK(hipStreamSynchronize(ctx->stream));  // rmap is pageable host memory
     }
-    qmcb_status st = QMCB_OK;
+    ctx->last_groups = G;
     if (G > 1) {
-      for (int g = 0; g < G; g++) {
-        const long first = (long)num * g / G, cnt = (long)num * (g + 1) / G - first;
-        const int *d_map = ctx->d_iota + first;
-        GroupView view(ctx, g, first, sp);
-        // stagger: group g's front half (proposal, Jastrow, basis, GEMM) of a
-        // step starts after group g-1's front half of the same step, so each
-        // group trails the one before it and its front overlaps that group's
-        // Sherman-Morrison pass; group 0 runs free
-        if (g > 0) QCHK(hipStreamWaitEvent(ctx->stream, ctx->gfront[g - 1], 0));
-        st = propose(step, m, d_map, (int)cnt);
-        if (st == QMCB_OK) st = DISPATCH(ctx, move_front(ctx, d_map, (int)cnt, m));
-        if (st == QMCB_OK) QCHK(hipEventRecord(ctx->gfront[g], ctx->stream));
-        if (st == QMCB_OK) {
-          MoveArgs a = DISPATCH(ctx, move_args(ctx, m, 1, none));
-          a.tau = tau;
-          if (m == 0) a.obs = ctx->d_obs;
-          st = DISPATCH(ctx, sm(ctx, a, d_map, (int)cnt));
+      qmcb_status gs = ensure_groups(ctx, G);
+      if (gs != QMCB_OK) return gs;
+      gs = groups_wait_main(ctx, G);
+      if (gs != QMCB_OK) return gs;
+    }
+    for (int s = 0; s < count; s++) {
+      const long step = step0 + s;
+      const int m = (int)(step % ctx->N);
+      const int sp = m < ctx->n[0] ? 0 : 1;
+      qmcb_status st = QMCB_OK;
+      if (G > 1) {
+        for (int g = 0; g < G; g++) {
+          const long first = (long)num * g / G, cnt = (long)num * (g + 1) / G - first;
+          const int *d_map = ctx->d_iota + first;
+          GroupView view(ctx, g, first, sp);
+          // stagger: group g's front half (proposal, Jastrow, basis, GEMM) of a
+          // step starts after group g-1's front half of the same step, so each
+          // group trails the one before it and its front overlaps that group's
+          // Sherman-Morrison pass; group 0 runs free
+          if (g > 0) QCHK(hipStreamWaitEvent(ctx->stream, ctx->gfront[g - 1], 0));
+          st = propose(step, m, d_map, (int)cnt);
+          if (st == QMCB_OK) st = DISPATCH(ctx, move_front(ctx, d_map, (int)cnt, m));
+          if (st == QMCB_OK) QCHK(hipEventRecord(ctx->gfront[g], ctx->stream));
+          if (st == QMCB_OK) {
+            MoveArgs a = DISPATCH(ctx, move_args(ctx, m, 1, none));
+            a.tau = tau;
+            if (m == 0) a.obs = ctx->d_obs;
+            st = DISPATCH(ctx, sm(ctx, a, d_map, (int)cnt));
+          }
+          if (st == QMCB_OK) st = staggered_refresh(ctx, first, first + cnt, sp);
+          if (st != QMCB_OK) return st;
         }
-        if (st == QMCB_OK && do_refresh) st = DISPATCH(ctx, refresh(ctx, d_map, (int)cnt, 1 << sp));
-        if (st != QMCB_OK) return st;
+        ctx->moves_of_spin[sp]++;
+        if (m == 0) {
+          // the sweep's observable row sums every walker in one fixed order
+          st = main_wait_groups(ctx, G);
+          if (st != QMCB_OK) return st;
+          TIMED("series", launch_series_reduce(ctx->d_obs, num,
+                                               ctx->d_series + 4L * ctx->series_len++,
+                                               ctx->stream));
+          st = groups_wait_main(ctx, G);
+          if (st != QMCB_OK) return st;
+        }
+        continue;
       }
-      if (m == 0) {
-        // the sweep's observable row sums every walker in one fixed order
-        st = main_wait_groups(ctx, G);
+      st = propose(step, m, nullptr, num);
+      if (st != QMCB_OK) return st;
+      st = DISPATCH(ctx, move_front(ctx, nullptr, num, m));
+      if (st != QMCB_OK) return st;
+      if (ctx->multi) {
+        // several determinants: stage every determinant, combine and accept on
+        // the device, then commit the accepted rows
+        st = DISPATCH(ctx, mdet_stage(ctx, nullptr, num, m, 1, none, tau,
+                                      m == 0 ? ctx->d_obs : nullptr));
         if (st != QMCB_OK) return st;
+        st = DISPATCH(ctx, mdet_commit(ctx, nullptr, ctx->d_mask, num, m));
+      } else {
+        MoveArgs a = DISPATCH(ctx, move_args(ctx, m, 1, none));
+        a.tau = tau;
+        if (m == 0) a.obs = ctx->d_obs;
+        st = DISPATCH(ctx, sm(ctx, a, nullptr, num));
+      }
+      if (st != QMCB_OK) return st;
+      if (m == 0)
         TIMED("series", launch_series_reduce(ctx->d_obs, num,
-                                             ctx->d_series + 4L * ctx->series_len++,
-                                             ctx->stream));
-        st = groups_wait_main(ctx, G);
-        if (st != QMCB_OK) return st;
-      }
-      continue;
-    }
-    st = propose(step, m, nullptr, num);
-    if (st != QMCB_OK) return st;
-    st = DISPATCH(ctx, move_front(ctx, nullptr, num, m));
-    if (st != QMCB_OK) return st;
-    if (ctx->multi) {
-      // several determinants: stage every determinant, combine and accept on
-      // the device, then commit the accepted rows
-      st = DISPATCH(ctx, mdet_stage(ctx, nullptr, num, m, 1, none, tau,
-                                    m == 0 ? ctx->d_obs : nullptr));
+                                             ctx->d_series + 4L * ctx->series_len++, ctx->stream));
+      st = staggered_refresh(ctx, 0, num, sp);
       if (st != QMCB_OK) return st;
-      st = DISPATCH(ctx, mdet_commit(ctx, nullptr, ctx->d_mask, num, m));
-    } else {
-      MoveArgs a = DISPATCH(ctx, move_args(ctx, m, 1, none));
-      a.tau = tau;
-      if (m == 0) a.obs = ctx->d_obs;
-      st = DISPATCH(ctx, sm(ctx, a, nullptr, num));
+      ctx->moves_of_spin[sp]++;
     }
-    if (st != QMCB_OK) return st;
-    if (m == 0)
-      TIMED("series", launch_series_reduce(ctx->d_obs, num, ctx->d_series + 4L * ctx->series_len++,
-                                           ctx->stream));
-    if (do_refresh) {
-      st = DISPATCH(ctx, refresh(ctx, nullptr, num, 1 << sp));
-      if (st != QMCB_OK) return st;
+    if (G > 1) {
+      qmcb_status gs = main_wait_groups(ctx, G);
+      if (gs != QMCB_OK) return gs;
     }
-  }
-  if (G > 1) {
-    qmcb_status gs = main_wait_groups(ctx, G);
-    if (gs != QMCB_OK) return gs;
-  }
-  return QMCB_OK;
+    return QMCB_OK;
   };
   // walker groups: single determinant only; at least 256 walkers per group
   auto clamp_groups = [&](int g) {
@@ -1482,12 +1545,15 @@ qmcb_status qmcb_vmc_steps(qmcb_ctx *ctx, int32_t num, int64_t first_step, int32
   int G = clamp_groups(ctx->opt.walker_groups);
   if (ctx->opt.walker_groups < 0) {
     // chosen once per context (again when the batch changes by more than 2x)
-    // by timing kTune steps of each candidate count on the real walkers, in
-    // this call and, if it is too short, the next ones (one group until the
-    // choice is made); the count changes no result bit, so the timed steps
-    // are ordinary steps.  A timed window holds no re-inversion: the steps up
-    // to one run untimed first.
-    const int kTune = ctx->opt.refresh_interval > 1 ? std::min(16, ctx->opt.refresh_interval - 1) : 16;
+    // by timing windows of kTune steps of each candidate count on the real
+    // walkers, in this call and, if it is too short, the next ones (one group
+    // until the choice is made).  One untimed round of windows comes first
+    // (it absorbs the GEMM tuning of each candidate's group batch and the
+    // first launches), then two timed rounds, interleaved, keeping each
+    // candidate's faster window.  The count changes no result bit, so the
+    // windows are ordinary steps; every window carries its share of the
+    // staggered refresh.
+    constexpr int kTune = 16;
     auto valid = [&]() {
       return ctx->auto_groups > 0 && num <= 2 * ctx->auto_groups_num &&
              2 * num >= ctx->auto_groups_num;
@@ -1497,41 +1563,23 @@ qmcb_status qmcb_vmc_steps(qmcb_ctx *ctx, int32_t num, int64_t first_step, int32
       const int c = clamp_groups(g);
       if (std::find(cand.begin(), cand.end(), c) == cand.end()) cand.push_back(c);
     }
-    if (!valid() && cand.size() == 1) {
+    const int nc = (int)cand.size(), windows = 3 * nc;
+    if (!valid() && nc == 1) {
       ctx->auto_groups = cand[0];
       ctx->auto_groups_num = num;
     } else if (!valid()) {
-      // one timed window per candidate, in this call or in the next ones
       if (ctx->tune_num != num) {
         ctx->tune_num = num;
         ctx->tune_q = 0;
       }
-      auto refresh_within = [&](long s0, int k) {
-        if (ctx->opt.refresh_interval <= 0) return -1;
-        long upd[2] = {ctx->sm_updates[0], ctx->sm_updates[1]};
-        for (int j = 0; j < k; j++) {
-          const int m = (int)((s0 + j) % ctx->N), sp = m < ctx->n[0] ? 0 : 1;
-          if (++upd[sp] >= ctx->opt.refresh_interval) return j;
-        }
-        return -1;
-      };
       qmcb_status ts = ensure_groups(ctx, cand.back());
       hipEvent_t e0 = nullptr, e1 = nullptr;
       if (ts == QMCB_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
         ts = QMCB_DEVICE_ERROR;
-      while (ts == QMCB_OK && ctx->tune_q < (int)cand.size()) {
-        const int g = cand[ctx->tune_q];
-        bool room = true;
-        for (int j; ts == QMCB_OK && room && (j = refresh_within(first_step + done, kTune)) >= 0;) {
-          room = done + j + 1 + kTune <= n_steps;
-          if (room) {
-            ts = run(first_step + done, j + 1, g);
-            done += j + 1;
-          }
-        }
-        if (ts != QMCB_OK || !room || done + kTune > n_steps) break;
+      while (ts == QMCB_OK && ctx->tune_q < windows && done + kTune <= n_steps) {
+        const int c = ctx->tune_q % nc, round = ctx->tune_q / nc;
         if (hipEventRecord(e0, ctx->stream) != hipSuccess) { ts = QMCB_DEVICE_ERROR; break; }
-        ts = run(first_step + done, kTune, g);
+        ts = run(first_step + done, kTune, cand[c]);
         done += kTune;
         float ms = 0.0f;
         if (ts == QMCB_OK && (hipEventRecord(e1, ctx->stream) != hipSuccess ||
@@ -1539,7 +1587,9 @@ qmcb_status qmcb_vmc_steps(qmcb_ctx *ctx, int32_t num, int64_t first_step, int32
                               hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
           ts = QMCB_DEVICE_ERROR;
         if (ts != QMCB_OK) break;
-        ctx->tune_ms[ctx->tune_q++] = ms;
+        if (round == 1) ctx->tune_ms[c] = ms;
+        if (round == 2) ctx->tune_ms[c] = std::min(ctx->tune_ms[c], ms);
+        ctx->tune_q++;
       }
       if (e0) (void)hipEventDestroy(e0);
       if (e1) (void)hipEventDestroy(e1);
@@ -1547,9 +1597,9 @@ qmcb_status qmcb_vmc_steps(qmcb_ctx *ctx, int32_t num, int64_t first_step, int32
         if (ts == QMCB_DEVICE_ERROR && ctx->err.empty()) ctx->err = "walker-group tuning failed";
         return ts;
       }
-      if (ctx->tune_q == (int)cand.size()) {  // every candidate timed: keep the fastest
+      if (ctx->tune_q == windows) {  // every candidate timed twice: keep the fastest
         int best = 0;
-        for (int q = 1; q < (int)cand.size(); q++)
+        for (int q = 1; q < nc; q++)
           if (ctx->tune_ms[q] < ctx->tune_ms[best]) best = q;
         ctx->auto_groups = cand[best];
         ctx->auto_groups_num = num;
@@ -1558,7 +1608,7 @@ qmcb_status qmcb_vmc_steps(qmcb_ctx *ctx, int32_t num, int64_t first_step, int32
     }
     G = valid() ? clamp_groups(ctx->auto_groups) : 1;
   }
-  {
+  if (done < n_steps) {
     qmcb_status rs = run(first_step + done, n_steps - done, G);
     if (rs != QMCB_OK) return rs;
   }
@@ -1615,6 +1665,7 @@ static CopyArgs walker_copy_args(qmcb_ctx *ctx) {
   add(ctx->Ven, 8);
   add(ctx->cur_el, 8);
   add(ctx->weight, 8);
+  add(ctx->age, 4);
   return c;
 }
 
@@ -1638,6 +1689,8 @@ qmcb_status qmcb_dmc_init(qmcb_ctx *ctx, int32_t num) {
   QCHK(hipStreamSynchronize(ctx->stream));
   ctx->dmc_ready = true;
   ctx->have_e_trial = false;
+  ctx->est_we = ctx->est_w = 0.0;
+  ctx->est_n = 0;
   return QMCB_OK;
 }
 
@@ -1661,8 +1714,11 @@ qmcb_status qmcb_dmc_steps(qmcb_ctx *ctx, int64_t first_step, int32_t n_steps,
   QCHK(hipMemsetAsync(ctx->acc_prop_el, 0, sizeof(double) * W, ctx->stream));
   QCHK(hipMemsetAsync(ctx->acc_w, 0, sizeof(double) * W, ctx->stream));
   if (!ctx->have_e_trial) {
+    // energy_estimated and energy_estimated_original start at the trial
+    // energy (QMCFlags.cpp:918-919)
     ctx->e_trial = prm->e_trial;
-    ctx->e_est = prm->e_trial;  // energy_estimated starts at the trial energy
+    ctx->e_est = prm->e_trial;
+    ctx->e_orig = prm->e_trial;
     ctx->have_e_trial = true;
   }
   {
@@ -1674,22 +1730,42 @@ qmcb_status qmcb_dmc_steps(qmcb_ctx *ctx, int64_t first_step, int32_t n_steps,
     }
   }
   const int target = prm->target_walkers > 0 ? prm->target_walkers : (int)ctx->active.size();
-  std::mt19937_64 rng(ctx->opt.seed ^ (0x9E3779B97F4A7C15ULL * (uint64_t)(first_step + 1)));
-  std::uniform_real_distribution<double> unif(0.0, 1.0);
+  const int K = ctx->opt.refresh_interval;
+  // the fusion draws are counter-based, keyed by (seed, step, fusion index),
+  // so a run split over several calls draws the same numbers as one call
+  auto fuse_uniform = [&](long step, int k) {
+    const u4 ctr = {(uint32_t)step, (uint32_t)((uint64_t)step >> 32), 0xFFFFFFFFu, (uint32_t)k};
+    const u4 r = philox4x32(ctr, (uint32_t)ctx->opt.seed, (uint32_t)(ctx->opt.seed >> 32));
+    return u01(r.x, r.y);
+  };
   qmcb_dmc_stats st{};
-  std::vector<double> wts(W), acc_e(W), acc_w(W);
+  std::vector<double> wts(W), row(4);
   const CopyArgs cargs = walker_copy_args(ctx);
   long moves = 0;
   // walker groups (single determinant): as in qmcb_vmc_steps, over the active
-  // list; walker_groups < 0 uses the count a qmcb_vmc_steps call chose
+  // list; walker_groups < 0 uses the count a qmcb_vmc_steps call chose, one
+  // group before one was chosen
   const int want = ctx->opt.walker_groups >= 0 ? ctx->opt.walker_groups : ctx->auto_groups;
   const int G = ctx->multi ? 1 : std::max(1, std::min<int>(want, qmcb_ctx::kMaxGroups));
   if (G > 1) {
     qmcb_status gs = ensure_groups(ctx, G);
     if (gs != QMCB_OK) return gs;
-    gs = groups_wait_main(ctx, G);
+  }
+  if (K > 0) {
+    qmcb_status ps = refresh_plan(ctx, ctx->active.data(), (int)ctx->active.size());
+    if (ps != QMCB_OK) return ps;
+    QCHK(hipStreamSynchronize(ctx->stream));  // rmap is pageable host memory
+  }
+  if (G > 1) {
+    qmcb_status gs = groups_wait_main(ctx, G);
     if (gs != QMCB_OK) return gs;
   }
+  WalkCtl walk{};
+  walk.age = ctx->age;
+  walk.old_walker = ctx->opt.old_walker_acceptance;
+  walk.rel_cutoff = prm->rel_cutoff;
+  walk.e_orig = ctx->e_orig;
+  walk.branch_threshold = prm->branch_threshold;
   for (int s = 0; s < n_steps; s++) {
     const long step = first_step + s;
     const int m = (int)(step % ctx->N);
@@ -1699,13 +1775,13 @@ qmcb_status qmcb_dmc_steps(qmcb_ctx *ctx, int64_t first_step, int32_t n_steps,
       return QMCB_BAD_STATE;
     }
     const int sp = m < ctx->n[0] ? 0 : 1;
-    bool do_refresh = false;
-    if (ctx->opt.refresh_interval > 0 && ++ctx->sm_updates[sp] >= ctx->opt.refresh_interval) {
-      ctx->sm_updates[sp] = 0;
-      do_refresh = true;
-    }
-    // one group's step over cnt rows of the active list (g < 0: no groups)
-    auto group_step = [&](const int *d_map, int cnt, int g) -> qmcb_status {
+    // the reference's iteration = step - equilibration_steps; the stricter
+    // deletion rules hold while iteration < -100 (QMCWalker.cpp:1207)
+    walk.equilibrating = step < (long)prm->equilibration_steps - 100 ? 1 : 0;
+    // one group's step over the active-list rows [first, first + cnt) (g < 0:
+    // no groups)
+    auto group_step = [&](long first, int cnt, int g) -> qmcb_status {
+      const int *d_map = ctx->d_active + first;
       ProposeArgs pa{};
       pa.N = ctx->N;
       pa.moved = m;
@@ -1727,6 +1803,7 @@ qmcb_status qmcb_dmc_steps(qmcb_ctx *ctx, int64_t first_step, int32_t n_steps,
       if (gst != QMCB_OK) return gst;
       if (g >= 0) QCHK(hipEventRecord(ctx->gfront[g], ctx->stream));
       qmcb_results none{};
+      const int reweight = (m == 0 && step >= prm->reweight_after) ? 1 : 0;
       if (ctx->multi) {
         // several determinants: stage every determinant, combine, accept
         // and reweight on the device, commit the accepted rows
@@ -1736,68 +1813,81 @@ qmcb_status qmcb_dmc_steps(qmcb_ctx *ctx, int64_t first_step, int32_t n_steps,
         ma.e_est = ctx->e_est;
         ma.energy_mod = prm->energy_modification;
         ma.tau_eff = prm->tau;
-        ma.reweight = (m == 0 && step >= prm->reweight_after) ? 1 : 0;
+        ma.reweight = reweight;
+        ma.walk = walk;
         ma.obs = m == 0 ? ctx->d_obs : nullptr;
         gst = DISPATCH(ctx, mdet_stage_args(ctx, ma, d_map, cnt));
         if (gst != QMCB_OK) return gst;
         gst = DISPATCH(ctx, mdet_commit(ctx, d_map, ctx->d_mask, cnt, m));
-        if (gst != QMCB_OK) return gst;
-        if (do_refresh) gst = DISPATCH(ctx, refresh(ctx, d_map, cnt, 1 << sp));
-        return gst;
+      } else {
+        MoveArgs a = DISPATCH(ctx, move_args(ctx, m, 2, none));
+        a.tau = prm->tau;
+        a.e_trial = ctx->e_trial;
+        a.e_est = ctx->e_est;
+        a.energy_mod = prm->energy_modification;
+        a.tau_eff = prm->tau;
+        a.reweight = reweight;
+        a.walk = walk;
+        if (m == 0) a.obs = ctx->d_obs;
+        gst = DISPATCH(ctx, sm(ctx, a, d_map, cnt));
       }
-      MoveArgs a = DISPATCH(ctx, move_args(ctx, m, 2, none));
-      a.tau = prm->tau;
-      a.e_trial = ctx->e_trial;
-      a.e_est = ctx->e_est;
-      a.energy_mod = prm->energy_modification;
-      a.tau_eff = prm->tau;
-      a.reweight = (m == 0 && step >= prm->reweight_after) ? 1 : 0;
-      if (m == 0) a.obs = ctx->d_obs;
-      gst = DISPATCH(ctx, sm(ctx, a, d_map, cnt));
       if (gst != QMCB_OK) return gst;
-      if (do_refresh) gst = DISPATCH(ctx, refresh(ctx, d_map, cnt, 1 << sp));
-      return gst;
+      return staggered_refresh(ctx, first, first + cnt, sp);
     };
+    const int Geff = (G > 1 && pop / G >= 256) ? G : 1;
+    ctx->last_groups = Geff;
     qmcb_status sst = QMCB_OK;
-    if (G > 1 && pop / G >= 256) {
+    if (Geff > 1) {
       for (int g = 0; g < G; g++) {
         const long first = (long)pop * g / G, cnt = (long)pop * (g + 1) / G - first;
         GroupView view(ctx, g, first, sp);
         if (g > 0) QCHK(hipStreamWaitEvent(ctx->stream, ctx->gfront[g - 1], 0));
-        sst = group_step(ctx->d_active + first, (int)cnt, g);
+        sst = group_step(first, (int)cnt, g);
         if (sst != QMCB_OK) return sst;
       }
       sst = main_wait_groups(ctx, G);
       if (sst != QMCB_OK) return sst;
     } else {
-      sst = group_step(ctx->d_active, pop, -1);
+      sst = group_step(0, pop, -1);
       if (sst != QMCB_OK) return sst;
     }
+    ctx->moves_of_spin[sp]++;
     if (m == 0)
       TIMED("series", launch_series_reduce(ctx->d_obs, pop, ctx->d_series + 4L * ctx->series_len++,
                                            ctx->stream));
     moves += pop;
-    if (G > 1) {
-      // the groups of the next step start after everything queued so far
-      sst = groups_wait_main(ctx, G);
-      if (sst != QMCB_OK) return sst;
+    if (m != 0) {
+      if (G > 1) {
+        // the groups of the next step start after everything queued so far
+        sst = groups_wait_main(ctx, G);
+        if (sst != QMCB_OK) return sst;
+      }
+      continue;
     }
-    if (m != 0) continue;
-    // ---- branching once per electron sweep (after the electron-0 move)
+    // ---- branching once per electron sweep (after the electron-0 move,
+    // QMCRun.cpp:1195-1206): the weights and this sweep's observable row
     QCHK(hipMemcpyAsync(wts.data(), ctx->weight, sizeof(double) * W, hipMemcpyDeviceToHost,
                         ctx->stream));
-    QCHK(hipMemcpyAsync(acc_e.data(), ctx->acc_el, sizeof(double) * W, hipMemcpyDeviceToHost,
-                        ctx->stream));
-    QCHK(hipMemcpyAsync(acc_w.data(), ctx->acc_w, sizeof(double) * W, hipMemcpyDeviceToHost,
-                        ctx->stream));
+    QCHK(hipMemcpyAsync(row.data(), ctx->d_series + 4L * (ctx->series_len - 1),
+                        sizeof(double) * 4, hipMemcpyDeviceToHost, ctx->stream));
     QCHK(hipStreamSynchronize(ctx->stream));
+    // nonunitWeightBranching (QMCRun.cpp:739-811): split above the branching
+    // threshold, delete below 0.01, fuse pairs below the fusion threshold in
+    // list order
     std::vector<int> next, pairs, dead;
     next.reserve(pop * 2);
-    int fuse_pending = -1;
+    int fuse_pending = -1, n_fuse = 0;
     for (int idx = 0; idx < pop; idx++) {
       const int w = ctx->active[idx];
       const double wt = wts[w];
-      if (wt > prm->branch_threshold && !ctx->free_slots.empty()) {
+      if (wt == 0.0) st.n_deleted++;
+      if (wt > prm->branch_threshold) {
+        if (ctx->free_slots.empty()) {
+          // no slot left: the walker keeps its weight unsplit (counted)
+          st.n_split_skipped++;
+          next.push_back(w);
+          continue;
+        }
         wts[w] = wt / 2.0;
         const int d = ctx->free_slots.back();
         ctx->free_slots.pop_back();
@@ -1816,7 +1906,7 @@ qmcb_status qmcb_dmc_steps(qmcb_ctx *ctx, int64_t first_step, int32_t n_steps,
           fuse_pending = w;
         } else {
           const double w1 = wts[fuse_pending], w3 = w1 + wt;
-          if (unif(rng) < w1 / w3) {
+          if (fuse_uniform(step, n_fuse++) < w1 / w3) {
             wts[fuse_pending] = w3;
             next.push_back(fuse_pending);
             dead.push_back(w);
@@ -1843,17 +1933,36 @@ qmcb_status qmcb_dmc_steps(qmcb_ctx *ctx, int64_t first_step, int32_t n_steps,
       TIMED("copy_walkers", launch_copy_walkers(cargs, ctx->d_pairs, (int)pairs.size() / 2,
                                                 ctx->stream));
     }
-    QCHK(hipMemcpyAsync(ctx->d_active, ctx->active.data(), sizeof(int) * ctx->active.size(),
-                        hipMemcpyHostToDevice, ctx->stream));
+    if (!ctx->active.empty())
+      QCHK(hipMemcpyAsync(ctx->d_active, ctx->active.data(), sizeof(int) * ctx->active.size(),
+                          hipMemcpyHostToDevice, ctx->stream));
+    if (K > 0) {
+      qmcb_status ps = refresh_plan(ctx, ctx->active.data(), (int)ctx->active.size());
+      if (ps != QMCB_OK) return ps;
+    }
     QCHK(hipStreamSynchronize(ctx->stream));  // host vectors above must outlive the copies
-    // trial energy: E_T = E_est - pop_control * ln(W_total / target)
-    double se = 0.0, sw = 0.0, wtot = 0.0;
-    for (int i = 0; i < W; i++) { se += acc_e[i]; sw += acc_w[i]; }
+    if (G > 1) {
+      sst = groups_wait_main(ctx, G);
+      if (sst != QMCB_OK) return sst;
+    }
+    // energy estimate and trial energy (QMCManager.cpp:2247-2310), once per
+    // sweep: the sweep's sample row[0] / row[1] with weight row[1]
+    double wtot = 0.0;
     for (int w : ctx->active) wtot += wts[w];
-    if (sw > 0 && step >= prm->reweight_after) {
-      const double ratio = wtot / target;
-      const double lr = ratio > 1e-250 ? std::log(ratio) : 0.0;
-      ctx->e_est = se / sw;
+    const double ratio = wtot / target;
+    const double lr = ratio > 1e-250 ? std::log(ratio) : 0.0;
+    if (step < (long)prm->equilibration_steps) {
+      // equilibration: E_est stays the input estimate, E_T follows the weight
+      // change (UNR93 eq. 11)
+      ctx->e_est = ctx->e_orig;
+      ctx->e_trial = ctx->e_est - lr / prm->tau;
+    } else {
+      if (row[1] > 0.0) {
+        ctx->est_we += row[0];
+        ctx->est_w += row[1];
+        ctx->est_n++;
+      }
+      ctx->e_est = (ctx->est_n >= 100 && ctx->est_w > 0.0) ? ctx->est_we / ctx->est_w : ctx->e_orig;
       ctx->e_trial = ctx->e_est - prm->pop_control * lr;
     }
   }
@@ -1876,6 +1985,7 @@ qmcb_status qmcb_dmc_steps(qmcb_ctx *ctx, int64_t first_step, int32_t n_steps,
     st.n_moves = moves;
     st.population = (int)ctx->active.size();
     st.e_trial = ctx->e_trial;
+    st.e_estimated = ctx->e_est;
   }
   if (stats) *stats = st;
   tm_collect(ctx);
@@ -1928,7 +2038,7 @@ qmcb_status qmcb_set_walker_groups(qmcb_ctx *ctx, int32_t groups) {
 
 qmcb_status qmcb_get_walker_groups(const qmcb_ctx *ctx, int32_t *groups) {
   if (!ctx || !groups) return QMCB_INVALID_ARG;
-  *groups = ctx->opt.walker_groups >= 0 ? ctx->opt.walker_groups : ctx->auto_groups;
+  *groups = ctx->last_groups;
   return QMCB_OK;
 }
 

@@ -38,6 +38,9 @@ struct DevSystem {
 };
 
 // ---- Pade u(r) = P(r)/Q(r), P(0)=0, Q(0)=1 (PadeCorrelationFunction.cpp:64-83)
+// The quotient-rule derivatives follow from u Q = P differentiated once and
+// twice: u' = (P' - u Q') / Q, u'' = (P'' - 2 u' Q' - u Q'') / Q, so one
+// reciprocal of Q serves all three.
 __device__ __forceinline__ void pade_eval(const qmcb_pade &f, double r, double &u,
                                           double &du, double &d2u) {
   // Horner for value, first and second derivative (Polynomial.cpp:127-149)
@@ -63,9 +66,54 @@ __device__ __forceinline__ void pade_eval(const qmcb_pade &f, double r, double &
     }
     bpp *= 2.0;
   }
-  u = a / b;
-  du = ap / b - a * bp / b / b;
-  d2u = app / b - 2 * ap * bp / b / b + 2 * a * bp * bp / b / b / b - a * bpp / b / b;
+  const double ib = 1.0 / b;
+  u = a * ib;
+  du = (ap - u * bp) * ib;
+  d2u = (app - 2.0 * du * bp - u * bpp) * ib;
+}
+
+// ---- walker bookkeeping of the reference's QMCWalker ------------------------
+// Acceptance of an old walker (QMCWalker.cpp:1578-1600): after more than
+// old_walker consecutive rejected moves the probability is raised by
+// 1.1^(age - old_walker), and set to 1 beyond 500 more.
+__device__ __forceinline__ double old_walker_boost(double p, int age, int old_walker) {
+  if (old_walker <= 0) return p;
+  const int a = age - old_walker;
+  if (a > 500) return 1.0;
+  if (a > 0) return p * pow(1.1, (double)a);
+  return p;
+}
+
+// DMC weight after one sweep's reweighting factor dW (QMCWalker.cpp:1180-1395):
+// an old walker decays by 0.9 instead of dW; then the reference's deletion
+// rules (weight 0, removed at the next branching).  Equilibration
+// (iteration < -100): heavy (> 2 branch_threshold), light (< 0.1), dW outside
+// [0.5, 1.5], a bad energy, or age >= 30.  Production: dW^(age+1) > 15, weight
+// > 50 or a bad energy.  A bad energy is |E_L - e_orig| / |e_orig| > rel_cutoff
+// (rel_cutoff <= 0: no energy rule); age is the count of consecutive
+// rejected moves after this move.
+struct WalkCtl {
+  int *age;              // [Wc] consecutive rejected moves (null: no age rules)
+  int old_walker;        // old_walker_acceptance_parameter (<= 0: off)
+  int equilibrating;     // the equilibration rules apply at this reweighting
+  double rel_cutoff;
+  double e_orig;         // energy_estimated_original
+  double branch_threshold;
+};
+
+__device__ inline double reweight_walker(double wt, double dW, int age, double el,
+                                         const WalkCtl &c) {
+  wt = (c.old_walker > 0 && age - c.old_walker > 0) ? wt * 0.9 : wt * dW;
+  const bool bad_energy = c.rel_cutoff > 0.0 && fabs((el - c.e_orig) / c.e_orig) > c.rel_cutoff;
+  if (c.equilibrating) {
+    if (wt > 2.0 * c.branch_threshold || wt < 0.1 || !(dW <= 1.5 && dW >= 0.5) || bad_energy ||
+        age >= 30)
+      return 0.0;
+    return wt;
+  }
+  const double growth = pow(dW, age + 1.0);
+  if (growth > 15.0 || dW != dW || wt > 50.0 || bad_energy) return 0.0;
+  return wt;
 }
 
 // ---- wavefront reductions (64 lanes) ---------------------------------------

@@ -130,6 +130,7 @@ struct MoveArgs {
   int reweight;          // reweight this step (electron 0 moved)
   int energy_mod;        // 0: energy_modification_type none, 1: umrigar93
   double e_est;          // energy_estimated of the umrigar93 modification
+  WalkCtl walk;          // walker age, old-walker acceptance, deletion rules
   // observable sample of this step (null except once per sweep):
   // obs[3 b + {0,1,2}] = {p E_L(R') + (1 - p) E_L(R), weight, p}
   double *obs;
@@ -176,6 +177,7 @@ struct MdetArgs {
   int reweight;
   int energy_mod;        // 0: energy_modification_type none, 1: umrigar93
   double e_est;
+  WalkCtl walk;          // walker age, old-walker acceptance, deletion rules
 };
 template <typename TS, typename TO>
 hipError_t launch_mdet_move(const MdetArgs &a, const int *walkers, int num, hipStream_t st);
@@ -220,6 +222,7 @@ struct AssembleArgs {
   double dt, umrigar_a;
   qmcb_results out;      // device arrays indexed by batch row
   double *cur_el;        // [Wc] current E_L (resident)
+  int *age;              // [Wc] walker age, reset for a fresh walker (may be null)
 };
 hipError_t launch_assemble(const AssembleArgs &a, const int *walkers, int num, hipStream_t st);
 

@@ -811,12 +811,15 @@ sm_move_kernel(MoveArgs A, const int *walkers, int num) {
   // ratio (QMCWalker.cpp:1578-1685): p = |Psi'/Psi|^2 G(R<-R')/G(R'<-R);
   // DMC additionally rejects node crossings (fixed node, :1641-1646)
   const double logpsi_old = ld_s + ld_o + A.U[w];
+  const int age_old = A.walk.age ? A.walk.age[w] : 0;
   double p = sing ? 0.0
                   : exp(2.0 * (logpsi_new - logpsi_old) - (s_bc[2] - s_bc[1]) / (2.0 * tau));
   if (!(p == p)) p = 0.0;
+  else if (!sing) p = old_walker_boost(p, age_old, A.walk.old_walker);
   if (A.mode == 2 && ratio < 0) p = 0.0;
   if (p > 1.0) p = 1.0;
   const bool accept = p > A.unif[b];
+  const int age_new = accept ? 0 : age_old + 1;  // QMCWalker.cpp:1656-1672
   const double el_old = A.cur_el[w];
   __syncthreads();  // every read of the current state is done
   const int base = flush ? 0 : dc;
@@ -857,6 +860,7 @@ sm_move_kernel(MoveArgs A, const int *walkers, int num) {
   }
   if (t == 0) {
     const double el = accept ? EL : el_old;
+    if (A.walk.age) A.walk.age[w] = age_new;
     double wt = 1.0;
     if (A.mode == 2) {
       // umrigar93_probability_weighted reweighting once per electron sweep
@@ -876,7 +880,8 @@ sm_move_kernel(MoveArgs A, const int *walkers, int num) {
           st_ = A.e_trial - EL;
         }
         if (!(p > 0.0)) st_ = 0.0;
-        wt *= exp(A.tau_eff * (p * 0.5 * (so_ + st_) + (1.0 - p) * so_));
+        const double dW = exp(A.tau_eff * (p * 0.5 * (so_ + st_) + (1.0 - p) * so_));
+        wt = reweight_walker(wt, dW, age_new, el, A.walk);
         A.weight[w] = wt;
       }
       A.acc_w[w] += wt;

@@ -125,3 +125,88 @@ def test_dmc_walker_groups_bit_identical():
     assert out[1][0] == out[0][0]
     for k in (1, 2, 3, 4):
         np.testing.assert_array_equal(out[1][k], out[0][k])
+
+
+def _h2_dmc(ctx, W0, R0):
+    res = ctx.evaluate(np.arange(W0), R0)
+    ctx.dmc_init(W0)
+    return float(np.mean(res["e_local"]))
+
+
+def test_dmc_split_calls_equal_one_call():
+    """The energy estimate (weighted mean of the production sweeps' samples,
+    fed back from the 100th sample on, QMCManager.cpp:2247-2261), the
+    fusion draws and the staggered refresh carry across calls: four calls of
+    50 sweeps end with the same walkers, weights, estimates and per-sweep
+    series as one call of 200 sweeps."""
+    packed, _, _ = load("h2_631gss")
+    W0, cap = 512, 1536
+    R0 = configs(packed, W0, 46)
+    N = R0.shape[1]
+    out = []
+    for calls in (1, 4):
+        print("case dmc split calls", calls, flush=True)
+        with Context(packed, precision="fp64", max_walkers=cap, seed=12, refresh_interval=8,
+                     replace_cusps=True) as ctx:
+            e0 = _h2_dmc(ctx, W0, R0)
+            step, rows, et = 0, [], []
+            per = 200 * N // calls
+            for c in range(calls):
+                st = ctx.dmc_steps(step, per, 0.01, e0, target=W0, branch_threshold=1.2,
+                                   fusion_threshold=0.8)
+                step += per
+                rows.append(ctx.sample_series())
+                et.append((st.e_trial, st.e_estimated))
+            slots, w = ctx.dmc_population()
+            out.append((slots, w, ctx.positions(cap)[slots], np.concatenate(rows), et[-1],
+                        st.population))
+    assert out[0][4][1] != e0  # the estimate moved off E_0 after 100 samples
+    for k in range(4):
+        np.testing.assert_array_equal(out[1][k], out[0][k])
+    assert out[1][4] == out[0][4] and out[1][5] == out[0][5]
+
+
+def test_dmc_deletion_rule_removes_bad_energies():
+    """rel_cutoff (QMCWalker.cpp:1270-1290, 1386-1393): at the first
+    reweighting (step 100) a walker whose energy lies further than
+    rel_cutoff |E_0| from E_0 gets weight 0 and the branching removes it.  A
+    cutoff below every walker's deviation deletes all of them (and the next
+    call reports the dead population); the reference's 100 deletes none of
+    the H2 walkers."""
+    packed, _, _ = load("h2_631gss")
+    W0, cap = 256, 768
+    R0 = configs(packed, W0, 47)
+    for cut, expect in ((1e-12, W0), (100.0, 0)):
+        print("case dmc rel_cutoff", cut, flush=True)
+        with Context(packed, precision="fp64", max_walkers=cap, seed=13,
+                     replace_cusps=True) as ctx:
+            e0 = _h2_dmc(ctx, W0, R0)
+            st = ctx.dmc_steps(0, 101, 0.01, e0, target=W0, rel_cutoff=cut)
+            assert st.n_deleted == expect, (cut, st.n_deleted)
+            if expect == W0:
+                assert st.population == 0 and st.n_killed == W0
+                with pytest.raises(Exception, match="died out"):
+                    ctx.dmc_steps(101, 2, 0.01, e0, target=W0, rel_cutoff=cut)
+            else:
+                assert st.population > 0.5 * W0
+
+
+def test_dmc_reports_splits_without_a_free_slot():
+    """A walker above the branching threshold with every slot in use keeps its
+    weight unsplit and is counted in n_split_skipped (the reference always
+    splits, QMCRun.cpp:748-757); with room the same sweep splits them."""
+    packed, _, _ = load("h2_631gss")
+    W0 = 256
+    R0 = configs(packed, W0, 48)
+    got = []
+    for cap in (W0, 2 * W0):
+        print("case dmc split capacity", cap, flush=True)
+        with Context(packed, precision="fp64", max_walkers=cap, seed=14,
+                     replace_cusps=True) as ctx:
+            e0 = _h2_dmc(ctx, W0, R0)
+            st = ctx.dmc_steps(0, 1, 0.01, e0, target=W0, branch_threshold=1.0001,
+                               fusion_threshold=0.5, reweight_after=0)
+            got.append((st.n_split, st.n_split_skipped))
+    print("splits", got, flush=True)
+    assert got[0][0] == 0 and got[0][1] > 0
+    assert got[1] == (got[0][1], 0)

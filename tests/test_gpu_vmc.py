@@ -147,9 +147,10 @@ def test_vmc_walker_groups_bit_identical(precision):
 
 def test_vmc_walker_groups_chosen_by_timing():
     """walker_groups = -1: the first long call times windows of 1, 2 and 3
-    groups (without re-inversions) inside its own steps and keeps the
-    fastest; the trajectory, the call's sums and the per-sweep series equal
-    the one-group run bit for bit, and the next call keeps the choice."""
+    groups inside its own steps (an untimed round, then two timed rounds)
+    and keeps the fastest; the trajectory, the call's sums and the
+    per-sweep series equal the one-group run bit for bit, and the next call
+    keeps the choice."""
     s = synth.make_system(1)
     W, N, steps = 1024, s.n_elec, 200
     R0 = synth.initial_walkers(s, W, 29)
@@ -159,7 +160,7 @@ def test_vmc_walker_groups_chosen_by_timing():
         with Context(s.pack(), precision="fp32", max_walkers=W, seed=6, refresh_interval=64,
                      walker_groups=groups) as ctx:
             ctx.evaluate(np.arange(W), R0)
-            assert ctx.walker_groups() == (1 if groups == 1 else 0)
+            assert ctx.walker_groups() == 0  # no steps yet
             st = ctx.vmc_steps(W, 0, steps, 0.02)
             chosen = ctx.walker_groups()
             st2 = ctx.vmc_steps(W, steps, N, 0.02)
@@ -171,3 +172,25 @@ def test_vmc_walker_groups_chosen_by_timing():
     np.testing.assert_array_equal(out[1][0], out[0][0])
     assert out[1][1] == out[0][1]
     np.testing.assert_array_equal(out[1][2], out[0][2])
+
+
+def test_vmc_refresh_is_staggered():
+    """refresh_interval K: after each move of a spin the walkers whose slot
+    is congruent to that spin's move count mod K are refreshed (FP64
+    re-inversion and traces), so every step carries 1/K of the refresh work:
+    one re-inversion and one trace launch per step of W/K walkers."""
+    s = synth.make_system(1)
+    W, K, steps = 256, 16, 20
+    say("vmc staggered refresh", W, K)
+    R0 = synth.initial_walkers(s, W, 24)
+    with Context(s.pack(), precision="fp32", max_walkers=W, refresh_interval=K,
+                 seed=8) as ctx:
+        ctx.evaluate(np.arange(W), R0)
+        ctx.enable_timing(True)
+        ctx.vmc_steps(W, 0, steps, 0.02)
+        kt = ctx.kernel_times()
+        ctx.enable_timing(False)
+    say("kernels", {k: v["launches"] for k, v in kt.items()})
+    assert kt["lu_inverse"]["launches"] == steps
+    assert kt["traces_all"]["launches"] == steps
+    assert kt["sm_move"]["launches"] == steps

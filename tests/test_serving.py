@@ -68,6 +68,7 @@ def test_plan_resident_state_recount():
         # 4 energy accumulators, 3 observables, weight acc_w, ln|Psi|, 8 outputs
         per += 8 * (1 + 4 + 1 + 1 + 3 + 4 + 3 + 2 + 1 + 8)
         per += 4 * (1 + 2 + 1)                             # active, pairs, ids
+        per += 4 * 2                                       # age, refresh map
         per += 1 + 1 + 1                                   # mask, output sign, output singular flag
         per += 5 * 16 * tx                                 # orbital rows (64-aligned; 16 padded orbitals)
         per += 5 * kpad * tx                               # basis scratch rows (W points)
