@@ -108,11 +108,14 @@ typedef struct qmcb_options {
   int32_t refresh_interval; /* device VMC/DMC: each walker's Slater matrices of
                                a spin are refreshed to FP64 accuracy once every
                                K single-electron moves of that spin, staggered
-                               by slot (slot s at the moves c of its spin with
-                               c mod K = s mod K), so every step refreshes
-                               about 1/K of the walkers of the moved spin; 0 =
-                               never (the reference never re-inverts in
-                               single-electron mode)                          */
+                               by slot: the residues s mod K form phases of S
+                               consecutive residues, S = min(K, ceil(K * 1024
+                               / walkers)), and phase q is refreshed after the
+                               move c of the spin with c mod K = min(qS + S - 1,
+                               K - 1), so every S steps refresh about S/K of
+                               the walkers of the moved spin in one launch of
+                               at least 1024 walkers; 0 = never (the reference
+                               never re-inverts in single-electron mode)      */
   double dt;                /* time step used by the drift modification        */
   double umrigar_a;         /* umrigar93_equalelectrons_parameter              */
   uint64_t seed;            /* device RNG seed (vmc entry points)              */

@@ -82,7 +82,7 @@ struct qmcb_ctx {
   // built from the DMC active list (rplan_num = -1)
   int *d_rmap = nullptr;
   std::vector<int> rmap, rpos, roff;
-  int rplan_num = 0, rplan_K = 0;
+  int rplan_num = 0, rplan_K = 0, rplan_S = 1;
   long moves_of_spin[2] = {0, 0};  // single-electron moves of each spin so far
   double *Rin = nullptr;
   double *propR = nullptr, *gauss = nullptr, *unif = nullptr;
@@ -1362,44 +1362,56 @@ static qmcb_status series_reserve(qmcb_ctx *ctx, int64_t first_step, int32_t n_s
 }
 
 // Staggered refresh.  Slot s of a spin is refreshed (FP64 re-inversion and
-// traces) right after the single-electron moves c of that spin (counted over
-// the context's life) with c mod K = s mod K, on the stream of the walker
-// group that moved it: every step re-inverts about 1/K of the walkers of the
-// moved spin, so any window of steps carries its share of the refresh work.
-// The plan sorts the walker list the steps run over (slots = null: the
-// identity list of the VMC; else the DMC active list) by residue, then list
-// position: rmap[roff[j] .. roff[j + 1]) are the slots with s mod K = j and
-// rpos their list positions, so the share of a walker group (a contiguous
-// range of list positions) is a contiguous piece of rmap.  rmap is copied to
-// d_rmap; the caller synchronises the stream before rmap changes again.
+// traces) once every K single-electron moves of that spin (counted over the
+// context's life), on the stream of the walker group that moved it.  The K
+// residues s mod K are grouped into phases of S consecutive residues; the
+// slots of phase q = (s mod K) / S are refreshed right after the move c with
+// c mod K = min(q S + S - 1, K - 1), i.e. at the last move of their phase.
+// S is the smallest phase length whose refresh launches carry at least
+// kRefreshBatch walkers (one re-inversion launch costs about the same for a
+// few matrices as for a full wave of workgroups), so a window of S or more
+// steps carries its share of the refresh work and every walker still
+// re-inverts exactly once per K moves of a spin.  The plan sorts the walker
+// list the steps run over (slots = null: the identity list of the VMC; else
+// the DMC active list) by phase, then list position: rmap[roff[q] ..
+// roff[q + 1]) are the slots of phase q and rpos their list positions, so the
+// share of a walker group (a contiguous range of list positions) is a
+// contiguous piece of rmap.  rmap is copied to d_rmap; the caller
+// synchronises the stream before rmap changes again.
+constexpr int kRefreshBatch = 1024;
 static qmcb_status refresh_plan(qmcb_ctx *ctx, const int *slots, int num) {
   const int K = ctx->opt.refresh_interval;
+  const int S = std::max(1, std::min(K, (int)(((long)K * kRefreshBatch + std::max(num, 1) - 1) /
+                                              std::max(num, 1))));
+  const int P = (K + S - 1) / S;
   ctx->rmap.assign(std::max(num, 1), 0);
   ctx->rpos.assign(std::max(num, 1), 0);
-  ctx->roff.assign((size_t)K + 1, 0);
-  for (int i = 0; i < num; i++) ctx->roff[(slots ? slots[i] : i) % K + 1]++;
-  for (int j = 0; j < K; j++) ctx->roff[j + 1] += ctx->roff[j];
+  ctx->roff.assign((size_t)P + 1, 0);
+  for (int i = 0; i < num; i++) ctx->roff[((slots ? slots[i] : i) % K) / S + 1]++;
+  for (int q = 0; q < P; q++) ctx->roff[q + 1] += ctx->roff[q];
   std::vector<int> fill(ctx->roff.begin(), ctx->roff.end() - 1);
   for (int i = 0; i < num; i++) {
-    const int sl = slots ? slots[i] : i, q = fill[sl % K]++;
-    ctx->rmap[q] = sl;
-    ctx->rpos[q] = i;
+    const int sl = slots ? slots[i] : i, at = fill[(sl % K) / S]++;
+    ctx->rmap[at] = sl;
+    ctx->rpos[at] = i;
   }
   if (num > 0)
     QCHK(hipMemcpyAsync(ctx->d_rmap, ctx->rmap.data(), sizeof(int) * num, hipMemcpyHostToDevice,
                         ctx->stream));
   ctx->rplan_num = slots ? -1 : num;
   ctx->rplan_K = K;
+  ctx->rplan_S = S;
   return QMCB_OK;
 }
 
 // The refresh share, after a move of spin sp, of the walkers at list
-// positions [first, last).
+// positions [first, last): the phase whose last move this is, if any.
 static qmcb_status staggered_refresh(qmcb_ctx *ctx, long first, long last, int sp) {
-  const int K = ctx->rplan_K;
+  const int K = ctx->rplan_K, S = ctx->rplan_S;
   if (K <= 0) return QMCB_OK;
-  const int j = (int)(ctx->moves_of_spin[sp] % K);
-  const int *p0 = ctx->rpos.data() + ctx->roff[j], *p1 = ctx->rpos.data() + ctx->roff[j + 1];
+  const int c = (int)(ctx->moves_of_spin[sp] % K), q = c / S;
+  if (c != std::min(q * S + S - 1, K - 1)) return QMCB_OK;
+  const int *p0 = ctx->rpos.data() + ctx->roff[q], *p1 = ctx->rpos.data() + ctx->roff[q + 1];
   const int *b = std::lower_bound(p0, p1, (int)first), *e = std::lower_bound(b, p1, (int)last);
   if (e == b) return QMCB_OK;
   return DISPATCH(ctx, refresh(ctx, ctx->d_rmap + (b - ctx->rpos.data()), (int)(e - b), 1 << sp));

@@ -174,14 +174,19 @@ def test_vmc_walker_groups_chosen_by_timing():
     np.testing.assert_array_equal(out[1][2], out[0][2])
 
 
-def test_vmc_refresh_is_staggered():
-    """refresh_interval K: after each move of a spin the walkers whose slot
-    is congruent to that spin's move count mod K are refreshed (FP64
-    re-inversion and traces), so every step carries 1/K of the refresh work:
-    one re-inversion and one trace launch per step of W/K walkers."""
+@pytest.mark.parametrize("W,K,S", [(4096, 16, 4), (256, 16, 16)])
+def test_vmc_refresh_is_staggered(W, K, S):
+    """refresh_interval K: the slot residues s mod K are grouped into phases
+    of S residues, S the shortest phase whose refresh launch carries at least
+    1024 walkers (S = ceil(16 * 1024 / W) capped at K); phase q is refreshed
+    (FP64 re-inversion and traces) after the move c of the spin with c mod K
+    = min(qS + S - 1, K - 1).  20 steps of spin-up moves (n_up = 32) then
+    refresh at c = 3, 7, 11, 15, 19 for W = 4096 and at c = 15 for W = 256,
+    one re-inversion and one trace launch each."""
     s = synth.make_system(1)
-    W, K, steps = 256, 16, 20
+    steps = 20
     say("vmc staggered refresh", W, K)
+    expect = sum(1 for c in range(steps) if c % K == min((c % K) // S * S + S - 1, K - 1))
     R0 = synth.initial_walkers(s, W, 24)
     with Context(s.pack(), precision="fp32", max_walkers=W, refresh_interval=K,
                  seed=8) as ctx:
@@ -191,6 +196,7 @@ def test_vmc_refresh_is_staggered():
         kt = ctx.kernel_times()
         ctx.enable_timing(False)
     say("kernels", {k: v["launches"] for k, v in kt.items()})
-    assert kt["lu_inverse"]["launches"] == steps
-    assert kt["traces_all"]["launches"] == steps
+    assert expect == {4: 5, 16: 1}[S]
+    assert kt["lu_inverse"]["launches"] == expect
+    assert kt["traces_all"]["launches"] == expect
     assert kt["sm_move"]["launches"] == steps
