@@ -205,6 +205,23 @@ qmcb_status qmcb_evaluate_device(qmcb_ctx *ctx, int32_t num,
                                  const double *R_dev, int32_t whichE,
                                  qmcb_results *out_dev);
 
+/* qmcb_evaluate_device with whichE = -1 (all-electron) that allocates
+ * nothing: no hipMalloc, hipFree or allocating library call.  No counterpart
+ * in the reference; it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The basis values of each chunk of points go to a static
+ * device array of 64 MiB per GPU (held until the process ends) instead of
+ * the context's scratch; results equal qmcb_evaluate_device's.  Limit: one
+ * walker's electrons of a spin must fit, 5 * ceil32(n_basis) * (8 FP64 and
+ * fp32store, 4 FP32, 2 BF16) bytes each, else QMCB_INVALID_ARG.  Every call
+ * on one GPU shares that GPU's array: calls that overlap on one GPU (other
+ * streams or threads, any context) race and must be serialised by the
+ * caller.  Kernel timing must be off (QMCB_BAD_STATE otherwise).
+ * Asynchronous on the context's stream. */
+qmcb_status qmcb_evaluate_device_noalloc(qmcb_ctx *ctx, int32_t num,
+                                         const int32_t *walker_ids_dev,
+                                         const double *R_dev, qmcb_results *out_dev);
+
 /* Commit (mask[i]=1) or discard (mask[i]=0) the staged single-electron
  * proposal of walker_ids[i].  Replaces the reference's re-evaluation at the
  * old position after a rejected move (QMCWalker.cpp:254-266). */

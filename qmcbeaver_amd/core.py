@@ -20,7 +20,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("QMCB_LIB_PATH") or os.path.join(HERE, "libqmcb_amd.so")
 
 EXPORTS = ["qmcb_abi_version", "qmcb_create", "qmcb_destroy", "qmcb_last_error",
-           "qmcb_evaluate", "qmcb_evaluate_device", "qmcb_accept", "qmcb_refresh",
+           "qmcb_evaluate", "qmcb_evaluate_device", "qmcb_evaluate_device_noalloc", "qmcb_accept",
+           "qmcb_refresh",
            "qmcb_get_positions", "qmcb_vmc_steps", "qmcb_sync", "qmcb_kernel_times",
            "qmcb_enable_kernel_timing", "qmcb_dmc_init", "qmcb_dmc_steps",
            "qmcb_dmc_population", "qmcb_sample_series", "qmcb_property_create",
@@ -59,6 +60,8 @@ def lib():
     L.qmcb_evaluate.argtypes = [vp, C.c_int32, ip, dp, C.c_int32, C.POINTER(Results)]
     L.qmcb_evaluate_device.restype = C.c_int
     L.qmcb_evaluate_device.argtypes = [vp, C.c_int32, vp, vp, C.c_int32, C.POINTER(Results)]
+    L.qmcb_evaluate_device_noalloc.restype = C.c_int
+    L.qmcb_evaluate_device_noalloc.argtypes = [vp, C.c_int32, vp, vp, C.POINTER(Results)]
     L.qmcb_accept.restype = C.c_int
     L.qmcb_accept.argtypes = [vp, C.c_int32, ip, C.POINTER(C.c_uint8)]
     L.qmcb_refresh.restype = C.c_int

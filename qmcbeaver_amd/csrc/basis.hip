@@ -130,6 +130,31 @@ hipError_t launch_scatter_rows(double *R, const double *Rin, const int *map, int
   return hipGetLastError();
 }
 
+// Zero rows [B, ldk) of the five components of an X buffer whose point
+// columns are [0, ldp): the padded basis rows the orbital GEMM reads as k
+// values beyond B.  A buffer shared by contexts of different systems (the
+// no-allocation evaluate's scratch) holds another layout's values there.
+// Rows [B, ldk) of one component are contiguous, so the fill is five byte
+// ranges of (ldk - B) * ldp * elem bytes, written as 8-byte words.
+__global__ void zero_pad_rows_kernel(unsigned long long *X, long comp_words, long first_word,
+                                     long pad_words) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 5 * pad_words) return;
+  const long c = i / pad_words, k = i % pad_words;
+  X[c * comp_words + first_word + k] = 0ull;
+}
+
+hipError_t launch_zero_pad_rows(void *X, size_t elem, int B, int ldk, long ldp, hipStream_t st) {
+  const size_t comp_bytes = (size_t)ldk * ldp * elem, first = (size_t)B * ldp * elem;
+  const size_t pad = (size_t)(ldk - B) * ldp * elem;
+  if (pad == 0) return hipSuccess;
+  if (comp_bytes % 8 || first % 8 || pad % 8) return hipErrorInvalidValue;
+  const long words = (long)(pad / 8), tot = 5 * words;
+  hipLaunchKernelGGL(zero_pad_rows_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
+                     (unsigned long long *)X, (long)(comp_bytes / 8), (long)(first / 8), words);
+  return hipGetLastError();
+}
+
 template <typename TC, typename TS>
 hipError_t launch_basis(const DevSystem &sys, const PointSet &pts, TS *X, int ldk, long ldp,
                         hipStream_t st) {

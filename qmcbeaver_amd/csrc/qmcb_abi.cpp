@@ -12,6 +12,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <random>
 #include <string>
@@ -38,6 +39,22 @@ struct Timing {
 };
 
 }  // namespace
+
+// Scratch of qmcb_evaluate_device_noalloc: the basis values of a chunk of
+// points, in a file-scope device array of kScratchBytes, one copy on every
+// GPU the library runs on, held until the process ends.  In host code the
+// array's name is only a stand-in: its device address (different on each
+// device) comes from hipGetSymbolAddress, looked up once per device at
+// qmcb_create with that device current and kept in g_scratch_dev; kernels
+// receive that address, never the name.  Every call on a GPU shares that
+// GPU's array, so calls that overlap on one GPU (two streams, two threads)
+// race; the entry point zeroes what must start at zero (the padded basis
+// rows) on every call.
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_dev[kMaxDevices] = {};
 
 struct qmcb_ctx {
   int device = 0;
@@ -143,6 +160,7 @@ struct qmcb_ctx {
   int tune_num = 0, tune_q = 0;  // timing in progress: batch size, windows run so far
   float tune_ms[3] = {0, 0, 0};  // best window per candidate
   int last_groups = 0;           // groups the last vmc/dmc steps ran with
+  void *scratch_dev = nullptr;   // this device's address of `scratch`
   // timing
   bool timing = false;
   Timing tm;
@@ -379,12 +397,19 @@ qmcb_status tune_gemm(qmcb_ctx *ctx, int sp, int num, F &&gemm) {
 // TS: derivative storage, TX: basis/GEMM operand, TO: GEMM output
 template <typename TS, typename TX, typename TC, typename TO>
 struct Engine {
-  // all-electron: slots given by d_map (device) for batch rows [0, num)
-  static qmcb_status all_electron(qmcb_ctx *ctx, const int *d_map, int num) {
+  // all-electron: slots given by d_map (device) for batch rows [0, num); the
+  // basis values go to X with a point capacity of ldp (the context's own
+  // scratch unless the caller brings one)
+  static qmcb_status all_electron(qmcb_ctx *ctx, const int *d_map, int num, void *Xs = nullptr,
+                                  long ldp = 0) {
+    if (!Xs) {
+      Xs = ctx->X;
+      ldp = ctx->ldp;
+    }
     for (int sp = 0; sp < 2; sp++) {
       const int n = ctx->n[sp];
       if (n == 0) continue;
-      const int chunk = std::max<long>(1, std::min<long>(num, ctx->ldp / n));
+      const int chunk = std::max<long>(1, std::min<long>(num, ldp / n));
       if (!d_map && chunk < num) {
         ctx->err = "all-electron evaluate in chunks needs a walker map";
         return QMCB_INVALID_ARG;
@@ -393,14 +418,14 @@ struct Engine {
         const int cw = std::min(chunk, num - c0);
         PointSet pts{ctx->R + 3L * ctx->start[sp], (long)cw * n, n, 3L * ctx->N, 3,
                      d_map + c0};
-        TIMED("basis", (launch_basis<TC, TX>(ctx->dsys, pts, (TX *)ctx->X, ctx->kpad, ctx->ldp,
+        TIMED("basis", (launch_basis<TC, TX>(ctx->dsys, pts, (TX *)Xs, ctx->kpad, ldp,
                                             ctx->stream)));
         const long na = ctx->nact[sp];
         GemmOut go{ctx->dmat[sp], na * n, (long)n, (long)kDer * na * n, 1, n, d_map + c0,
                    ctx->dval[sp], na * n};
         TIMED("orbital_gemm", (launch_orbital_gemm<TX, TS>((const TX *)ctx->Ct[sp], ctx->npad[sp],
                                                           (int)na,
-                                                          (const TX *)ctx->X, ctx->kpad, ctx->ldp,
+                                                          (const TX *)Xs, ctx->kpad, ldp,
                                                           (long)cw * n, ctx->kpad, go,
                                                           ctx->stream)));
         if (ctx->cusp)
@@ -658,6 +683,20 @@ qmcb_status create_impl(const qmcb_system *s, const qmcb_options *o, qmcb_ctx *c
   QCHK(hipSetDevice(o->device));
   ctx->device = o->device;
   QCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  {
+    // the device address of the no-allocation scratch, once per device
+    if (o->device < 0 || o->device >= kMaxDevices) {
+      ctx->err = "device ordinal out of range";
+      return QMCB_INVALID_ARG;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[o->device]) {
+      void *ptr = nullptr;
+      QCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+      g_scratch_dev[o->device] = ptr;
+    }
+    ctx->scratch_dev = g_scratch_dev[o->device];
+  }
   {
     const size_t cap = plan_device_bytes(s, o);
     QCHK(hipMalloc((void **)&ctx->arena, cap));
@@ -1200,6 +1239,42 @@ qmcb_status qmcb_evaluate_device(qmcb_ctx *ctx, int32_t num, const int32_t *ids_
                           hipMemcpyDeviceToDevice, ctx->stream));
   }
   return evaluate_core(ctx, num, ids_dev, R_dev, whichE, *out_dev);
+}
+
+qmcb_status qmcb_evaluate_device_noalloc(qmcb_ctx *ctx, int32_t num, const int32_t *ids_dev,
+                                         const double *R_dev, qmcb_results *out_dev) {
+  if (!ctx || !out_dev || (num > 0 && (!ids_dev || !R_dev))) return QMCB_INVALID_ARG;
+  if (num < 0 || num > ctx->Wc) {
+    ctx->err = "num out of range";
+    return QMCB_INVALID_ARG;
+  }
+  if (ctx->timing) {
+    ctx->err = "qmcb_evaluate_device_noalloc: kernel timing creates events; switch it off";
+    return QMCB_BAD_STATE;
+  }
+  if (!ctx->scratch_dev) {
+    ctx->err = "no device address for the scratch array";
+    return QMCB_BAD_STATE;
+  }
+  // point capacity of the scratch for this system's basis (a multiple of 64,
+  // as the basis and GEMM kernels index whole 64-point tiles); one walker's
+  // electrons of a spin must fit
+  const size_t per_point = (size_t)kComp * ctx->kpad * ctx->tx;
+  const long cap = (long)(kScratchBytes / per_point) / 64 * 64;
+  const int nmax = std::max(ctx->n[0], ctx->n[1]);
+  if (cap < nmax) {
+    ctx->err = "qmcb_evaluate_device_noalloc: one walker's basis values (" +
+               std::to_string(per_point * nmax) + " bytes) exceed the " +
+               std::to_string(kScratchBytes) + "-byte scratch";
+    return QMCB_INVALID_ARG;
+  }
+  if (num == 0) return QMCB_OK;
+  QCHK(hipSetDevice(ctx->device));
+  QCHK(launch_scatter_rows(ctx->R, R_dev, ids_dev, num, ctx->N * 3, ctx->stream));
+  QCHK(launch_zero_pad_rows(ctx->scratch_dev, ctx->tx, ctx->B, ctx->kpad, cap, ctx->stream));
+  qmcb_status s = DISPATCH(ctx, all_electron(ctx, ids_dev, num, ctx->scratch_dev, cap));
+  if (s != QMCB_OK) return s;
+  return DISPATCH(ctx, assemble(ctx, ids_dev, num, *out_dev));
 }
 
 qmcb_status qmcb_accept(qmcb_ctx *ctx, int32_t num, const int32_t *ids, const uint8_t *mask) {

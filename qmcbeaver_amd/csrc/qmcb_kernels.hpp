@@ -44,6 +44,8 @@ hipError_t launch_cusp(const CuspDev &c, const PointSet &pts, const GemmOut &o, 
 // scatter batch rows of positions into resident slots: R[map[b]] = Rin[b]
 hipError_t launch_scatter_rows(double *R, const double *Rin, const int *map, int num, int row_len,
                                hipStream_t st);
+// zero rows [B, ldk) of the five components of X[5][ldk][ldp] (elem bytes each)
+hipError_t launch_zero_pad_rows(void *X, size_t elem, int B, int ldk, long ldp, hipStream_t st);
 
 // ---- kernel (a) ----
 template <typename TC, typename TS>
