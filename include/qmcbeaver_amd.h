@@ -146,6 +146,14 @@ typedef struct qmcb_options {
                                1.1^(age - K) (QMCWalker.cpp:1578-1600) and, in
                                DMC, its weight decays by 0.9 per sweep instead
                                of being reweighted (:1181-1184).  <= 0: off    */
+  int32_t all_electron_moves; /* 1: qmcb_vmc_steps moves every electron of a
+                               walker at each step (the reference's
+                               one_e_per_iter 0, its default; QMCWalker.cpp:
+                               179-198, 343-398, 756-780): the whole evaluate
+                               runs at the proposal and a rejected walker gets
+                               its saved record back (:259-262); one observable
+                               row per step.  0: one electron per step, moved
+                               in order (one_e_per_iter 1)                   */
 } qmcb_options;
 
 /* Caller-owned outputs, one row per evaluated walker.  Any pointer may be

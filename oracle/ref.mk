@@ -20,6 +20,10 @@ OBJS := $(patsubst $(REFROOT)/src/%.cpp,$(OBJDIR)/%.o,$(SRCS))
 
 all: $(OUT)/qmcb_ref_driver
 
+# the reference program itself (its main, QMcBeaver.cpp, over the same
+# objects): whole VMC/DMC runs of a .ckmf input, as the reference runs them
+qmcbeaver: $(OUT)/QMcBeaver.x
+
 $(OBJDIR)/%.o: $(REFROOT)/src/%.cpp
 	@mkdir -p $(OBJDIR)
 	$(CXX) $(REFFLAGS) -c $< -o $@
@@ -27,4 +31,7 @@ $(OBJDIR)/%.o: $(REFROOT)/src/%.cpp
 $(OUT)/qmcb_ref_driver: oracle/ref_driver.cpp $(OBJS)
 	$(CXX) $(REFFLAGS) oracle/ref_driver.cpp $(OBJS) -o $@
 
-.PHONY: all
+$(OUT)/QMcBeaver.x: $(REFROOT)/src/QMcBeaver.cpp $(OBJS)
+	$(CXX) $(REFFLAGS) $(REFROOT)/src/QMcBeaver.cpp $(OBJS) -o $@
+
+.PHONY: all qmcbeaver

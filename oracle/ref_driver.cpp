@@ -12,6 +12,7 @@
 //   qmcb_ref_driver eval <input.ckmf> <configs.bin> <out.bin>
 //   qmcb_ref_driver seq  <input.ckmf> <moves.bin>   <out.bin>
 //   qmcb_ref_driver vmc  <input.ckmf> <configs.bin> <n_moves> <tau> <seed>
+//   qmcb_ref_driver vmc_ae <input.ckmf> <configs.bin> <n_eq> <n_steps> <tau> <seed>
 //   qmcb_ref_driver ddda <series.bin> <split>
 //
 // configs.bin: int32 nconf, int32 N, double R[nconf][N][3]
@@ -312,6 +313,92 @@ int main(int argc, char **argv) {
                 "\"ddda_std\": %.12g}\n",
                 dt, (long long)nw * n_moves, esum / ((double)nw * n_moves), acc / ((double)nw * n_moves),
                 energy.getNumberSamples(), energy.getAverage(), energy.getStandardDeviation());
+  } else if (mode == "vmc_ae") {
+    // Importance-sampled all-electron VMC (one_e_per_iter 0) with the
+    // reference evaluate: every electron moves, R' = R + tau F(R) +
+    // sqrt(tau) xi (QMCWalker.cpp:343-398); the reverse Green's function
+    // over all electrons (:756-780); p = min(1, |Psi'/Psi|^2 G(R<-R') /
+    // G(R'<-R)) (:1578-1653); a rejected walker gets its saved record back
+    // (:259-262).  Statistics per step after n_eq steps: the energy sample
+    // sum_w (p E_L(R') + (1 - p) E_L(R)) / W through QMCProperty
+    // (QMCWalker.cpp:2096-2104, QMCRun.cpp:544-576), the mean acceptance
+    // probability p over walkers and steps, and the fraction of accepted
+    // moves.
+    int hdr[2];
+    if (std::fread(hdr, sizeof(int), 2, fin) != 2 || hdr[1] != N) return 3;
+    const int nw = hdr[0];
+    const int n_eq = std::atoi(argv[4]), n_steps = argc > 5 ? std::atoi(argv[5]) : 100;
+    const double tau = argc > 6 ? std::atof(argv[6]) : 0.1;
+    g_state = argc > 7 ? std::strtoull(argv[7], 0, 10) : 1;
+    std::vector<Array2D<double> > R(nw);
+    std::vector<QMCWalkerData *> wd(nw);
+    for (int w = 0; w < nw; w++) {
+      R[w].allocate(N, 3);
+      for (int e = 0; e < N; e++)
+        for (int k = 0; k < 3; k++) {
+          double v;
+          if (std::fread(&v, sizeof(double), 1, fin) != 1) return 4;
+          R[w](e, k) = v;
+        }
+      wd[w] = new QMCWalkerData();
+      wd[w]->initialize(3, 0, 0);
+      wd[w]->whichE = -1;
+      wd[w]->updateDistances(R[w]);
+      qmf.evaluate(R[w], *wd[w]);
+    }
+    QMCWalkerData trial;
+    trial.initialize(3, 0, 0);
+    trial.whichE = -1;
+    static QMCProperty energy, accp;  // static: zero-filled sums (see ddda_main)
+    double psum = 0.0, acc = 0.0, t0 = now();
+    const double sig = std::sqrt(tau);
+    Array2D<double> Rp(N, 3);
+    for (int it = 0; it < n_eq + n_steps; it++) {
+      double step_e = 0.0, step_p = 0.0;
+      for (int w = 0; w < nw; w++) {
+        double fwd = 0.0;
+        for (int e = 0; e < N; e++)
+          for (int k = 0; k < 3; k++) {
+            const double g = sig * gauss();
+            Rp(e, k) = R[w](e, k) + tau * wd[w]->modifiedGradPsiRatio(e, k) + g;
+            fwd += g * g;
+          }
+        trial = *wd[w];  // the walker record, as createChildWalkers copies it
+        trial.whichE = -1;
+        trial.updateDistances(Rp);
+        qmf.evaluate(Rp, trial);
+        double rev = 0.0;
+        for (int e = 0; e < N; e++)
+          for (int k = 0; k < 3; k++) {
+            const double t = R[w](e, k) - Rp(e, k) - tau * trial.modifiedGradPsiRatio(e, k);
+            rev += t * t;
+          }
+        double lo, so, ln, sn;
+        log_qmcdouble(wd[w]->psi, lo, so);
+        log_qmcdouble(trial.psi, ln, sn);
+        double p = trial.singular ? 0.0 : std::exp(2.0 * (ln - lo) - (rev - fwd) / (2.0 * tau));
+        if (!(p == p)) p = 0.0;
+        if (p > 1.0) p = 1.0;
+        step_e += p * trial.localEnergy + (1.0 - p) * wd[w]->localEnergy;
+        step_p += p;
+        if (p > unif()) {
+          *wd[w] = trial;
+          R[w] = Rp;
+          if (it >= n_eq) acc += 1.0;
+        }
+      }
+      if (it >= n_eq) {
+        energy.newSample(step_e / nw, (double)nw);
+        accp.newSample(step_p / nw, (double)nw);
+        psum += step_p;
+      }
+    }
+    const double dt = now() - t0, moves = (double)nw * n_steps;
+    std::printf("{\"seconds\": %.6f, \"walker_moves\": %.0f, \"acceptance_probability\": %.8f, "
+                "\"accepted_fraction\": %.8f, \"ddda_samples\": %lu, \"ddda_energy\": %.12f, "
+                "\"ddda_std\": %.12g, \"acceptance_probability_std\": %.12g}\n",
+                dt, moves, psum / moves, acc / moves, energy.getNumberSamples(),
+                energy.getAverage(), energy.getStandardDeviation(), accp.getStandardDeviation());
   } else {
     return 2;
   }

@@ -11,7 +11,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libqmcb_amd.so")
 SOURCES = ["basis.hip", "orbital_gemm.hip", "slater.hip", "jastrow.hip", "cusp.hip", "multidet.hip",
-           "qmcb_abi.cpp", "qmcb_stats.cpp", "qmcb_cusp.cpp"]
+           "walk_ae.hip", "qmcb_abi.cpp", "qmcb_stats.cpp", "qmcb_cusp.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -134,7 +134,8 @@ def cusp_fit(packed: Dict, spin: int = 0) -> np.ndarray:
 
 def _options(precision, max_walkers, drift_mod=QMCB_DRIFT_NONE, dt=0.1, umrigar_a=0.5,
              refresh_interval=0, seed=12345, device=0, replace_cusps=False,
-             basis_scratch_bytes=0, walker_groups=0, old_walker_acceptance=50) -> Options:
+             basis_scratch_bytes=0, walker_groups=0, old_walker_acceptance=50,
+             all_electron_moves=False) -> Options:
     o = Options()
     o.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
     o.max_walkers = max_walkers
@@ -148,6 +149,7 @@ def _options(precision, max_walkers, drift_mod=QMCB_DRIFT_NONE, dt=0.1, umrigar_
     o.basis_scratch_bytes = int(basis_scratch_bytes)
     o.walker_groups = int(walker_groups)
     o.old_walker_acceptance = int(old_walker_acceptance)
+    o.all_electron_moves = 1 if all_electron_moves else 0
     return o
 
 
@@ -186,17 +188,19 @@ class Context:
                  drift_mod: int = QMCB_DRIFT_NONE, dt: float = 0.1, umrigar_a: float = 0.5,
                  refresh_interval: int = 0, seed: int = 12345, device: int = 0,
                  replace_cusps: bool = False, basis_scratch_bytes: int = 0,
-                 walker_groups: int = 0, old_walker_acceptance: int = 50):
+                 walker_groups: int = 0, old_walker_acceptance: int = 50,
+                 all_electron_moves: bool = False):
         """old_walker_acceptance: the reference's old_walker_acceptance_parameter
         (acceptance boost and DMC weight decay of walkers stuck for more than
-        this many moves); 0 switches the rule off."""
+        this many moves); 0 switches the rule off.  all_electron_moves: every
+        electron moves at each vmc step (the reference's one_e_per_iter 0)."""
         self._buf = SystemBuffers(packed)
         self.N = self._buf.N
         self.n_up = int(packed["n_up"])
         self.max_walkers = max_walkers
         o = _options(precision, max_walkers, drift_mod, dt, umrigar_a, refresh_interval, seed,
                      device, replace_cusps, basis_scratch_bytes, walker_groups,
-                     old_walker_acceptance)
+                     old_walker_acceptance, all_electron_moves)
         self._opt = o
         h = C.c_void_p()
         st = lib().qmcb_create(C.byref(self._buf.struct), C.byref(o), C.byref(h))

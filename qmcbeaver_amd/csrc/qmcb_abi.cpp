@@ -94,6 +94,11 @@ struct qmcb_ctx {
   double *U = nullptr, *Ux = nullptr, *Uxx = nullptr, *Vee = nullptr, *Ven = nullptr;
   double *cur_el = nullptr;
   int *age = nullptr;  // [Wc] consecutive rejected moves (QMCWalker::age)
+  // all-electron moves (opt.all_electron_moves): the walker record at R --
+  // modified drift [Wc][N][3], ln|Psi|, sign, |modified drift| / |drift| and
+  // the move's forward Green's-function sum [Wc]; Rin holds R during a move
+  double *ae_drift = nullptr, *ae_logpsi = nullptr, *ae_sgn = nullptr, *ae_mr = nullptr,
+         *ae_fwd = nullptr;
   // staggered refresh: the walker list sorted by slot residue mod K (see
   // refresh_plan), valid for the VMC identity list of rplan_num walkers, or
   // built from the DMC active list (rplan_num = -1)
@@ -443,7 +448,11 @@ struct Engine {
     return QMCB_OK;
   }
 
-  static qmcb_status assemble(qmcb_ctx *ctx, const int *d_map, int num, const qmcb_results &out) {
+  // what: bit 0 writes each walker's current E_L (cur_el), bit 1 resets its
+  // age (a fresh walker); dt < 0: the context's time step for the drift
+  // modification
+  static qmcb_status assemble(qmcb_ctx *ctx, const int *d_map, int num, const qmcb_results &out,
+                              double dt = -1.0, int what = 3) {
     if (ctx->multi) {
       MdetArgs ma = mdet_args(ctx, -1, 0, out);
       TIMED("mdet_assemble", launch_mdet_assemble(ma, d_map, num, ctx->stream));
@@ -456,11 +465,11 @@ struct Engine {
     aa.R = ctx->R; aa.U = ctx->U; aa.Ux = ctx->Ux; aa.Uxx = ctx->Uxx;
     aa.Vee = ctx->Vee; aa.Ven = ctx->Ven;
     aa.drift_mod = ctx->opt.drift_mod;
-    aa.dt = ctx->opt.dt;
+    aa.dt = dt < 0 ? ctx->opt.dt : dt;
     aa.umrigar_a = ctx->opt.umrigar_a;
     aa.out = out;
-    aa.cur_el = ctx->cur_el;
-    aa.age = ctx->age;
+    aa.cur_el = (what & 1) ? ctx->cur_el : nullptr;
+    aa.age = (what & 2) ? ctx->age : nullptr;
     TIMED("assemble", launch_assemble(aa, d_map, num, ctx->stream));
     return QMCB_OK;
   }
@@ -906,6 +915,13 @@ qmcb_status create_impl(const qmcb_system *s, const qmcb_options *o, qmcb_ctx *c
   QCHK(dalloc(ctx, &ctx->d_pairs, 2 * W));
   QCHK(dalloc(ctx, &ctx->d_ids, W));
   QCHK(dalloc(ctx, &ctx->d_mask, W));
+  if (o->all_electron_moves) {
+    QCHK(dalloc(ctx, &ctx->ae_drift, W * N * 3));
+    QCHK(dalloc(ctx, &ctx->ae_logpsi, W));
+    QCHK(dalloc(ctx, &ctx->ae_sgn, W));
+    QCHK(dalloc(ctx, &ctx->ae_mr, W));
+    QCHK(dalloc(ctx, &ctx->ae_fwd, W));
+  }
   // basis scratch: at least one point per walker; all-electron chunks reuse it
   const size_t per_point = (size_t)kComp * ctx->kpad * ctx->tx;
   const size_t budget = o->basis_scratch_bytes > 0 ? (size_t)o->basis_scratch_bytes
@@ -1085,6 +1101,10 @@ size_t plan_device_bytes(const qmcb_system *s, const qmcb_options *o) {
   add(2 * W, 4);
   add(W, 4);
   add(W, 1);
+  if (o->all_electron_moves) {
+    add(W * N * 3, 8);                       // all-electron record: drift,
+    for (int i = 0; i < 4; i++) add(W, 8);  // ln|Psi|, sign, ratio, forward sum
+  }
   const size_t per_point = (size_t)kComp * kpad * tx;
   const size_t budget = o->basis_scratch_bytes > 0 ? (size_t)o->basis_scratch_bytes
                                                    : (size_t)1 << 31;
@@ -1418,8 +1438,9 @@ static qmcb_status main_wait_groups(qmcb_ctx *ctx, int G) {
   return QMCB_OK;
 }
 
-static qmcb_status series_reserve(qmcb_ctx *ctx, int64_t first_step, int32_t n_steps) {
-  const int64_t N = ctx->N;
+static qmcb_status series_reserve(qmcb_ctx *ctx, int64_t first_step, int32_t n_steps,
+                                  int64_t N = 0) {
+  if (N <= 0) N = ctx->N;
   const int64_t first = ((first_step + N - 1) / N) * N;  // first multiple of N >= first_step
   const int64_t last = first_step + n_steps;             // exclusive
   const int64_t rows = last > first ? (last - first + N - 1) / N : 0;
@@ -1492,6 +1513,60 @@ static qmcb_status staggered_refresh(qmcb_ctx *ctx, long first, long last, int s
   return DISPATCH(ctx, refresh(ctx, ctx->d_rmap + (b - ctx->rpos.data()), (int)(e - b), 1 << sp));
 }
 
+// All-electron moves (opt.all_electron_moves).  ae_args fills the record and
+// accumulator pointers of `a`, whose mode (1 VMC, 2 DMC) and DMC fields the
+// caller sets.  ae_begin takes each walker's record at its current
+// positions from the resident state (which describes R between calls);
+// ae_step makes one move of the walkers d_map[0, num): the proposal written
+// over R (R saved in Rin), the ordinary all-electron evaluate and assembly
+// at it, the Metropolis decision with restore and the observable sample in
+// d_obs; ae_end evaluates every walker at R again, since a rejected
+// walker's Slater and Jastrow state describes its proposal.
+static void ae_args(qmcb_ctx *ctx, AeArgs &a, double tau) {
+  a.N = ctx->N;
+  a.R = ctx->R;
+  a.Rsave = ctx->Rin;
+  a.drift = ctx->ae_drift;
+  a.logpsi = ctx->ae_logpsi;
+  a.sgn = ctx->ae_sgn;
+  a.mr = ctx->ae_mr;
+  a.fwd = ctx->ae_fwd;
+  a.cur_el = ctx->cur_el;
+  a.prop = ctx->dout;
+  a.tau = tau;
+  a.seed = ctx->opt.seed;
+  a.unif = ctx->unif;
+  a.acc_el = ctx->acc_el; a.acc_el2 = ctx->acc_el2; a.acc_n = ctx->acc_n;
+  a.acc_prop_el = ctx->acc_prop_el;
+  a.walk.age = ctx->age;
+  a.walk.old_walker = ctx->opt.old_walker_acceptance;
+}
+
+static qmcb_status ae_begin(qmcb_ctx *ctx, const int *d_map, int num, const AeArgs &a) {
+  qmcb_status s = DISPATCH(ctx, assemble(ctx, d_map, num, ctx->dout, a.tau, 0));
+  if (s != QMCB_OK) return s;
+  AeArgs a0 = a;
+  a0.mode = 0;
+  TIMED("ae_accept", launch_ae_accept(a0, d_map, num, ctx->stream));
+  return QMCB_OK;
+}
+
+static qmcb_status ae_step(qmcb_ctx *ctx, const int *d_map, int num, const AeArgs &a) {
+  TIMED("ae_propose", launch_ae_propose(a, d_map, num, ctx->stream));
+  qmcb_status s = DISPATCH(ctx, all_electron(ctx, d_map, num));
+  if (s != QMCB_OK) return s;
+  s = DISPATCH(ctx, assemble(ctx, d_map, num, ctx->dout, a.tau, 0));
+  if (s != QMCB_OK) return s;
+  TIMED("ae_accept", launch_ae_accept(a, d_map, num, ctx->stream));
+  return QMCB_OK;
+}
+
+static qmcb_status ae_end(qmcb_ctx *ctx, const int *d_map, int num) {
+  qmcb_status s = DISPATCH(ctx, all_electron(ctx, d_map, num));
+  if (s != QMCB_OK) return s;
+  return DISPATCH(ctx, assemble(ctx, d_map, num, ctx->dout, -1.0, 1));
+}
+
 qmcb_status qmcb_vmc_steps(qmcb_ctx *ctx, int32_t num, int64_t first_step, int32_t n_steps,
                            double tau, qmcb_vmc_stats *stats) {
   if (!ctx || num < 1 || num > ctx->Wc || n_steps < 0 || first_step < 0 || !(tau > 0))
@@ -1507,10 +1582,11 @@ qmcb_status qmcb_vmc_steps(qmcb_ctx *ctx, int32_t num, int64_t first_step, int32
   QCHK(hipMemsetAsync(ctx->acc_el2, 0, sizeof(double) * num, ctx->stream));
   QCHK(hipMemsetAsync(ctx->acc_n, 0, sizeof(double) * num, ctx->stream));
   QCHK(hipMemsetAsync(ctx->acc_prop_el, 0, sizeof(double) * num, ctx->stream));
+  const bool ae = ctx->opt.all_electron_moves != 0;
   {
-    qmcb_status rs = series_reserve(ctx, first_step, n_steps);
+    qmcb_status rs = series_reserve(ctx, first_step, n_steps, ae ? 1 : 0);
     if (rs != QMCB_OK) return rs;
-    if (ctx->multi) {
+    if (ctx->multi && !ae) {
       rs = ensure_staging(ctx);
       if (rs != QMCB_OK) return rs;
     }
@@ -1630,7 +1706,28 @@ qmcb_status qmcb_vmc_steps(qmcb_ctx *ctx, int32_t num, int64_t first_step, int32
   };
   int done = 0;
   int G = clamp_groups(ctx->opt.walker_groups);
-  if (ctx->opt.walker_groups < 0) {
+  if (ae) {
+    // all-electron moves: one group, one observable row per step
+    qmcb_status as = ensure_groups(ctx, 1);
+    if (as != QMCB_OK) return as;
+    AeArgs a{};
+    ae_args(ctx, a, tau);
+    a.mode = 1;
+    a.obs = ctx->d_obs;
+    as = ae_begin(ctx, ctx->d_iota, num, a);
+    for (int32_t i = 0; as == QMCB_OK && i < n_steps; i++) {
+      a.step = first_step + i;
+      as = ae_step(ctx, ctx->d_iota, num, a);
+      if (as == QMCB_OK) {
+        TIMED("series", launch_series_reduce(ctx->d_obs, num,
+                                             ctx->d_series + 4L * ctx->series_len++, ctx->stream));
+      }
+    }
+    if (as == QMCB_OK) as = ae_end(ctx, ctx->d_iota, num);
+    if (as != QMCB_OK) return as;
+    ctx->last_groups = 1;
+    done = n_steps;
+  } else if (ctx->opt.walker_groups < 0) {
     // chosen once per context (again when the batch changes by more than 2x)
     // by timing windows of kTune steps of each candidate count on the real
     // walkers, in this call and, if it is too short, the next ones (one group
@@ -1753,6 +1850,12 @@ static CopyArgs walker_copy_args(qmcb_ctx *ctx) {
   add(ctx->cur_el, 8);
   add(ctx->weight, 8);
   add(ctx->age, 4);
+  if (ctx->ae_drift) {  // the all-electron record
+    add(ctx->ae_drift, N * 3 * 8);
+    add(ctx->ae_logpsi, 8);
+    add(ctx->ae_sgn, 8);
+    add(ctx->ae_mr, 8);
+  }
   return c;
 }
 
@@ -1808,10 +1911,11 @@ qmcb_status qmcb_dmc_steps(qmcb_ctx *ctx, int64_t first_step, int32_t n_steps,
     ctx->e_orig = prm->e_trial;
     ctx->have_e_trial = true;
   }
+  const bool ae = ctx->opt.all_electron_moves != 0;
   {
-    qmcb_status rs = series_reserve(ctx, first_step, n_steps);
+    qmcb_status rs = series_reserve(ctx, first_step, n_steps, ae ? 1 : 0);
     if (rs != QMCB_OK) return rs;
-    if (ctx->multi) {
+    if (ctx->multi && !ae) {
       rs = ensure_staging(ctx);
       if (rs != QMCB_OK) return rs;
     }
@@ -1833,12 +1937,12 @@ qmcb_status qmcb_dmc_steps(qmcb_ctx *ctx, int64_t first_step, int32_t n_steps,
   // list; walker_groups < 0 uses the count a qmcb_vmc_steps call chose, one
   // group before one was chosen
   const int want = ctx->opt.walker_groups >= 0 ? ctx->opt.walker_groups : ctx->auto_groups;
-  const int G = ctx->multi ? 1 : std::max(1, std::min<int>(want, qmcb_ctx::kMaxGroups));
+  const int G = (ctx->multi || ae) ? 1 : std::max(1, std::min<int>(want, qmcb_ctx::kMaxGroups));
   if (G > 1) {
     qmcb_status gs = ensure_groups(ctx, G);
     if (gs != QMCB_OK) return gs;
   }
-  if (K > 0) {
+  if (K > 0 && !ae) {
     qmcb_status ps = refresh_plan(ctx, ctx->active.data(), (int)ctx->active.size());
     if (ps != QMCB_OK) return ps;
     QCHK(hipStreamSynchronize(ctx->stream));  // rmap is pageable host memory
@@ -1853,9 +1957,25 @@ qmcb_status qmcb_dmc_steps(qmcb_ctx *ctx, int64_t first_step, int32_t n_steps,
   walk.rel_cutoff = prm->rel_cutoff;
   walk.e_orig = ctx->e_orig;
   walk.branch_threshold = prm->branch_threshold;
+  // all-electron moves: every step moves every electron, reweights (from
+  // reweight_after on; QMCWalker.cpp:226-228) and branches
+  AeArgs ae_a{};
+  if (ae) {
+    ae_args(ctx, ae_a, prm->tau);
+    ae_a.mode = 2;
+    ae_a.weight = ctx->weight;
+    ae_a.acc_w = ctx->acc_w;
+    ae_a.energy_mod = prm->energy_modification;
+    ae_a.tau_eff = prm->tau;
+    ae_a.obs = ctx->d_obs;
+    if (!ctx->active.empty()) {
+      qmcb_status as = ae_begin(ctx, ctx->d_active, (int)ctx->active.size(), ae_a);
+      if (as != QMCB_OK) return as;
+    }
+  }
   for (int s = 0; s < n_steps; s++) {
     const long step = first_step + s;
-    const int m = (int)(step % ctx->N);
+    const int m = ae ? 0 : (int)(step % ctx->N);
     const int pop = (int)ctx->active.size();
     if (pop == 0) {
       ctx->err = "DMC population died out";
@@ -1924,7 +2044,15 @@ qmcb_status qmcb_dmc_steps(qmcb_ctx *ctx, int64_t first_step, int32_t n_steps,
     const int Geff = (G > 1 && pop / G >= 256) ? G : 1;
     ctx->last_groups = Geff;
     qmcb_status sst = QMCB_OK;
-    if (Geff > 1) {
+    if (ae) {
+      ae_a.step = step;
+      ae_a.reweight = step >= prm->reweight_after ? 1 : 0;
+      ae_a.e_trial = ctx->e_trial;
+      ae_a.e_est = ctx->e_est;
+      ae_a.walk = walk;
+      sst = ae_step(ctx, ctx->d_active, pop, ae_a);
+      if (sst != QMCB_OK) return sst;
+    } else if (Geff > 1) {
       for (int g = 0; g < G; g++) {
         const long first = (long)pop * g / G, cnt = (long)pop * (g + 1) / G - first;
         GroupView view(ctx, g, first, sp);
@@ -1938,7 +2066,7 @@ qmcb_status qmcb_dmc_steps(qmcb_ctx *ctx, int64_t first_step, int32_t n_steps,
       sst = group_step(0, pop, -1);
       if (sst != QMCB_OK) return sst;
     }
-    ctx->moves_of_spin[sp]++;
+    if (!ae) ctx->moves_of_spin[sp]++;
     if (m == 0)
       TIMED("series", launch_series_reduce(ctx->d_obs, pop, ctx->d_series + 4L * ctx->series_len++,
                                            ctx->stream));
@@ -2023,7 +2151,7 @@ qmcb_status qmcb_dmc_steps(qmcb_ctx *ctx, int64_t first_step, int32_t n_steps,
     if (!ctx->active.empty())
       QCHK(hipMemcpyAsync(ctx->d_active, ctx->active.data(), sizeof(int) * ctx->active.size(),
                           hipMemcpyHostToDevice, ctx->stream));
-    if (K > 0) {
+    if (K > 0 && !ae) {
       qmcb_status ps = refresh_plan(ctx, ctx->active.data(), (int)ctx->active.size());
       if (ps != QMCB_OK) return ps;
     }
@@ -2052,6 +2180,10 @@ qmcb_status qmcb_dmc_steps(qmcb_ctx *ctx, int64_t first_step, int32_t n_steps,
       ctx->e_est = (ctx->est_n >= 100 && ctx->est_w > 0.0) ? ctx->est_we / ctx->est_w : ctx->e_orig;
       ctx->e_trial = ctx->e_est - prm->pop_control * lr;
     }
+  }
+  if (ae && !ctx->active.empty()) {
+    qmcb_status as = ae_end(ctx, ctx->d_active, (int)ctx->active.size());
+    if (as != QMCB_OK) return as;
   }
   // statistics
   {

@@ -247,6 +247,33 @@ struct ProposeArgs {
 };
 hipError_t launch_propose(const ProposeArgs &a, const int *walkers, int num, hipStream_t st);
 
+// all-electron moves (walk_ae.hip; QMCWalker.cpp:343-398, 756-780,
+// 1578-1685): the walker record at the current positions, the proposal
+// written over R (R saved), the evaluate's outputs at the proposal by batch
+// row, Metropolis and restore
+constexpr uint32_t kAeTag = 0x40000000u;  // Philox counter word of these draws
+struct AeArgs {
+  int N;
+  double *R, *Rsave;     // [Wc][N][3]
+  double *drift;         // [Wc][N][3] modified drift at R
+  double *logpsi, *sgn, *mr, *fwd;  // [Wc]
+  double *cur_el;        // [Wc]
+  qmcb_results prop;     // evaluate outputs at the proposal, by batch row
+  double tau;
+  uint64_t seed;
+  long step;
+  double *unif;          // [Wc] by batch row
+  int mode;              // 0: take `prop` as the record; 1: VMC; 2: DMC
+  double *acc_el, *acc_el2, *acc_n, *acc_prop_el;
+  double *weight, *acc_w;
+  double e_trial, tau_eff, e_est;
+  int reweight, energy_mod;
+  WalkCtl walk;
+  double *obs;           // [num][3] per-step observable sample (may be null)
+};
+hipError_t launch_ae_propose(const AeArgs &a, const int *walkers, int num, hipStream_t st);
+hipError_t launch_ae_accept(const AeArgs &a, const int *walkers, int num, hipStream_t st);
+
 // commit staged single-electron proposals
 template <typename TS, typename TO>
 hipError_t launch_commit(const MoveArgs &a, const int *walkers, const uint8_t *mask, int num,
