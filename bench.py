@@ -286,10 +286,14 @@ def serving_run(precision, n_jobs, rate, seed, workers, capacity_gib):
 
 def batched_run(precision, n_mol, steps, warmup, seed, workers, tau):
     """Batched uneven DMC: n_mol molecules (8-200 electrons, B ~ 5 N,
-    256-4096 walkers each), DMC with tau, all molecules advanced together."""
+    256-4096 walkers each), DMC with tau and the reference's default cusp
+    replacement, drift and energy modification, all molecules advanced
+    together; the timed steps follow `warmup` steps (from 100 on, every
+    sweep reweights and branches)."""
     from qmcbeaver_amd import serving
     comp = serving.batched_composition(n_mol, seed)
-    with serving.BatchedDMC(comp, precision=precision, tau=tau, workers=workers) as b:
+    with serving.BatchedDMC(comp, precision=precision, tau=tau, workers=workers,
+                            replace_cusps=True) as b:
         fp = b.footprint()
         print(f"batched {precision}: {n_mol} molecules, {sum(c[1] for c in comp)} walkers, "
               f"projected {fp / 2**30:.1f} GiB", file=sys.stderr, flush=True)

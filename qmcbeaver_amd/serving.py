@@ -317,18 +317,21 @@ class BatchedDMC:
     on FP64 weights), one context and host thread per molecule."""
 
     def __init__(self, composition, precision: str = "fp32", tau: float = 0.01,
-                 workers: int = 16, basis_scratch_bytes: int = 256 << 20):
+                 workers: int = 16, basis_scratch_bytes: int = 256 << 20,
+                 replace_cusps: bool = False):
         self.comp = list(composition)
         self.tau = tau
         self.workers = max(1, min(int(workers), len(self.comp)))
         self.precision = precision
         self.scratch = basis_scratch_bytes
+        self.cusps = bool(replace_cusps)
         self.ctx: List[Optional[Context]] = [None] * len(self.comp)
         self.state = [dict(step=0, e_t=0.0, moves=0, pop=0) for _ in self.comp]
 
     def footprint(self) -> int:
         return sum(plan_device_bytes(job_molecule(ne, sd).pack(), precision=self.precision,
-                                     max_walkers=2 * W, basis_scratch_bytes=self.scratch)
+                                     max_walkers=2 * W, basis_scratch_bytes=self.scratch,
+                                     replace_cusps=self.cusps)
                    + CONTEXT_OVERHEAD for ne, W, sd in self.comp)
 
     def _pool(self, fn, items):
@@ -361,15 +364,18 @@ class BatchedDMC:
             s = job_molecule(ne, sd)
             c = Context(s.pack(), precision=self.precision, max_walkers=2 * W,
                         refresh_interval=64 if self.precision != "fp64" else 0, seed=sd,
-                        basis_scratch_bytes=self.scratch, drift_mod=DRIFT)
+                        basis_scratch_bytes=self.scratch, drift_mod=DRIFT,
+                        replace_cusps=self.cusps)
             self.ctx[i] = c
             res = c.evaluate(np.arange(W), synth.initial_walkers(s, W, sd))
             self.state[i]["e_t"] = initial_trial_energy(res)
             c.dmc_init(W)
         self._pool(one, range(len(self.comp)))
 
-    def advance(self, steps: int, rows: Optional[List] = None):
-        """`steps` single-electron DMC steps of every molecule."""
+    def advance(self, steps: int, rows: Optional[List] = None, stats: Optional[List] = None):
+        """`steps` single-electron DMC steps of every molecule; rows[i] gets
+        molecule i's (sum w E_L, sum w, population, E_T), stats[i] its
+        branching counts of these steps."""
         def one(i):
             st_i = self.state[i]
             W = self.comp[i][1]
@@ -381,6 +387,11 @@ class BatchedDMC:
             st_i["pop"] = int(st.population)
             if rows is not None:
                 rows[i] = (st.sum_w_e_local, st.sum_weights, int(st.population), st.e_trial)
+            if stats is not None:
+                stats[i] = {k: getattr(st, k) for k in (
+                    "n_split", "n_killed", "n_fused", "n_deleted", "n_split_skipped",
+                    "population", "total_weight", "e_trial", "e_estimated", "sum_accept",
+                    "n_moves")}
         self._pool(one, range(len(self.comp)))
 
     def close(self):
@@ -402,13 +413,18 @@ class BatchedDMC:
         self.advance(warmup)
         for s in self.state:
             s["moves"] = 0
+        stats: List = [None] * len(self.comp)
         t0 = time.perf_counter()
-        self.advance(steps)
+        self.advance(steps, stats=stats)
         wall = time.perf_counter() - t0
         moves = sum(s["moves"] for s in self.state)
+        branching = {k: int(sum(s[k] for s in stats))
+                     for k in ("n_split", "n_fused", "n_killed", "n_deleted", "n_split_skipped")}
         return {"molecules": len(self.comp), "steps": steps, "warmup": warmup,
+                "replace_cusps": self.cusps,
                 "electrons": [c[0] for c in self.comp],
                 "walkers": [c[1] for c in self.comp],
                 "population_end": [s["pop"] for s in self.state],
+                "branching_in_timed_steps": branching,
                 "wall_s": wall, "setup_s": t_setup, "walker_moves": moves,
                 "walker_moves_per_s": moves / wall}

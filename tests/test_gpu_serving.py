@@ -15,7 +15,16 @@ in the mix, and the serving loop's admission with real contexts.
   identical per-molecule results;
 * the serving loop with a capacity that forces queueing never holds more
   projected bytes than the capacity, and every job's context stayed within
-  its projection.
+  its projection;
+* at the two many-molecule workloads' full size: the 64-molecule batched
+  DMC composition (150 243 walkers, 8-198 electrons, the reference's default
+  cusp replacement, drift and energy modification) run past the unit-weight
+  phase (steps 100-499 reweight and branch) keeps every population alive
+  while splitting, fusing and deleting walkers, and three of its molecules
+  run alone give the same per-molecule results bit for bit; the 200-job
+  Poisson trace at 20 jobs/s (8-256 electrons, 1e3-1e5 walker-moves per job)
+  completes, and its largest jobs re-run alone give the same per-block
+  results bit for bit.
 """
 import os
 import tempfile
@@ -129,3 +138,65 @@ def test_server_capacity_forces_queueing():
     assert out["completed"] == len(jobs), out["failed"]
     assert srv.peak_bytes <= cap
     assert out["block_latency_ms"]["p99"] >= out["block_latency_ms"]["p50"] > 0
+
+
+CFG3_SEED = 0x51EED03
+
+
+def test_batched_dmc_config3_full_composition_past_step_100():
+    comp = serving.batched_composition(64, CFG3_SEED)
+    assert len(comp) == 64 and sum(c[1] for c in comp) == 150243
+    print("case batched DMC config 3: 64 molecules, cusp replacement, steps 0-499", flush=True)
+    rows, stats = [None] * 64, [None] * 64
+    with serving.BatchedDMC(comp, precision="fp32", workers=16, replace_cusps=True) as b:
+        b.setup()
+        b.advance(100)  # weights stay 1 for the first 100 steps
+        b.advance(400, rows, stats)
+    tot = {k: sum(st[k] for st in stats) for k in
+           ("n_split", "n_fused", "n_killed", "n_deleted", "n_split_skipped")}
+    W = np.array([c[1] for c in comp])
+    pop = np.array([st["population"] for st in stats])
+    branched = sum(1 for st in stats if st["n_split"] + st["n_fused"] + st["n_killed"] > 0)
+    print("branching", tot, "molecules branched", branched, "population/target min %.2f max %.2f"
+          % ((pop / W).min(), (pop / W).max()), flush=True)
+    assert tot["n_split"] > 0 and tot["n_fused"] > 0 and tot["n_killed"] > 0
+    assert branched == 64  # every molecule reweighted and branched after step 100
+    assert np.all(pop >= 0.5 * W) and np.all(pop <= 2 * W)
+    for sw_e, sw, p, e_t in rows:
+        assert sw > 0 and np.isfinite(sw_e / sw) and np.isfinite(e_t)
+    # isolation at full size: the smallest, the largest and the most populous
+    # molecule run by themselves
+    pick = sorted({0, int(np.argmax([c[0] for c in comp])), int(np.argmax(W))})
+    alone = [None] * len(pick)
+    with serving.BatchedDMC([comp[i] for i in pick], precision="fp32", workers=1,
+                            replace_cusps=True) as b:
+        b.setup()
+        b.advance(100)
+        b.advance(400, alone)
+    for k, i in enumerate(pick):
+        assert alone[k] == rows[i], i
+
+
+def test_serving_config4_trace_and_isolation():
+    jobs = serving.poisson_trace(200, 20.0, 20)  # the benchmark's trace
+    runner = serving.JobRunner(precision="fp32")
+    print("case serving config 4: 200 jobs at 20/s", flush=True)
+    srv = serving.Server(capacity_bytes=160 << 30, workers=8, runner=runner)
+    out = srv.run(jobs)
+    print("completed", out["completed"], "wall", out["wall_s"], "latency",
+          out["block_latency_ms"], "peak active", out["peak_active"], flush=True)
+    assert out["completed"] == 200, out["failed"]
+    assert out["rejected"] == [] and srv.peak_active >= 2
+    assert max(j.n_elec for j in jobs) >= 240
+    moves = {j.job_id: j.walkers * j.block_steps * j.blocks for j in jobs}
+    sel = {max((j for j in jobs if j.kind == k), key=lambda j: moves[j.job_id]).job_id
+           for k in ("vmc", "dmc")}
+    sel.add(max(jobs, key=lambda j: j.n_elec).job_id)
+    sel.add(max(jobs, key=lambda j: moves[j.job_id] * j.n_elec).job_id)
+    chosen = [j for j in jobs if j.job_id in sel]
+    print("re-run alone", [(j.job_id, j.kind, j.n_elec, j.walkers, j.blocks) for j in chosen],
+          flush=True)
+    alone = serving.run_alone(_fresh(chosen), runner)
+    for j in chosen:
+        assert len(j.results) == j.blocks
+        assert j.results == alone[j.job_id], j.job_id
