@@ -201,21 +201,6 @@ __device__ __forceinline__ void row16_argmax(double &best, int &bi) {
   take(dpp_f64<0x140>(best), dpp_i32<0x140>(bi));
 }
 
-// The same over the 64 lanes (all lanes active); every lane gets the winner.
-__device__ __forceinline__ void wave_argmax(double &best, int &bi) {
-  row16_argmax(best, bi);
-  double b0 = readlane_f64(best, 0);
-  int i0 = __builtin_amdgcn_readlane(bi, 0);
-#pragma unroll
-  for (int r = 16; r < 64; r += 16) {
-    const double br = readlane_f64(best, r);
-    const int ir = __builtin_amdgcn_readlane(bi, r);
-    if (br > b0 || (br == b0 && ir < i0)) { b0 = br; i0 = ir; }
-  }
-  best = b0;
-  bi = i0;
-}
-
 // ---- Philox4x32-10 (Salmon et al., SC'11) counter-based RNG ---------------
 struct u4 { uint32_t x, y, z, w; };
 

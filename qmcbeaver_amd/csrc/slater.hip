@@ -186,156 +186,376 @@ lu_inverse_kernel(SlaterArgs A, const int *walkers, int num) {
 }
 
 // ---------------------------------------------------------------------------
-// Row-resident Gauss-Jordan for NP in {64, 128}: row i of the matrix lives in
-// the registers of the lane pair (2i, 2i+1), NP/2 doubles each (half h holds
-// register positions h*NP/2 ...).  Pivot columns are processed in blocks of 16
-// with compile-time register indices; after each block every row is rotated
-// left by 16 positions (16 values cross between the partner lanes by
-// shuffle), so the next block's pivot columns are again positions 0..15 of
-// half 0 and no register array is indexed dynamically.  All rows share the
-// rotation state, so LDS rows (pivot row, swapped row) are exchanged in
-// position order.  Per pivot: a wavefront argmax of |a[k]| over rows >= k,
-// one row interchange through LDS, NP/2 FMAs per lane.  The inverse is written
-// through an LDS tile so that the column permutation of the interchanges
-// lands in coalesced stores of rows of A^-1(o, e).
+// Blocked Gauss-Jordan inverse on the FP64 matrix cores
+// (v_mfma_f64_16x16x4f64), 32 < n <= NP, NP in {64, 128}.  It inverts
+// B = A^T, whose rows are the stored rows (o, e) of dval, so the loads are
+// coalesced (inv(B) = inv(A)^T, det B = det A).  B, padded to NP with the
+// identity, stays in registers in the accumulator layout of the instruction
+// (element i of lane l of a 16x16 tile: row 4i + l/16, column l%16); wave v
+// of the four owns the 16-column tiles v, v + 4, ...  Columns are eliminated
+// in panels of 16 with partial pivoting and no row interchanges:
+//   1. the wave that owns the panel moves its NP x 16 block to one row per
+//      lane (two for NP = 128) and runs the 16 Gauss-Jordan steps of the
+//      panel alone: pivot = lowest row index among the largest |column| over
+//      rows not yet pivoted (wavefront max, then a ballot), the pivot row read
+//      across lanes, every other row reduced.  Pivot rows stay unscaled until
+//      the panel ends (a row is linear in its own scale, so only the pivot
+//      column entry, 1, and a pending factor 1/pivot differ), which keeps the
+//      steps free of branches.  Padded columns (>= n) are eliminated like the
+//      others: their pivot is their own identity row and changes nothing.
+//      This leaves XK, the panel columns of the final matrix;
+//   2. every wave copies the panel's pivot rows P (as they were before the
+//      panel) out of its other columns and zeroes them there;
+//   3. every other column becomes  B[:, j] <- B[:, j] + XK B_P[:, j]  (four
+//      MFMA per 16x16 tile), which is what the 16 sequential steps do to it.
+// Two workgroup barriers per panel.  With pivot row p_k for column k the
+// final X holds inv(B)[k][p_c] = X[p_k][c], so
+//   inv(o, e) = inv(B)[e][o] = X[p_e][c(o)],   c(o): the column pivoted on row o,
+// and det B = sign(k -> p_k) * prod_k pivot_k; the sign of the permutation is
+// the parity of its inversions, counted per pivot from the mask of rows
+// pivoted before it.  The operands of step 3 pass through LDS already in the
+// lane order of the instruction (A operand lane l: XK[16 rt + l%16][4 s + l/16];
+// B operand lane l: B_P[4 s + l/16][16 ct + l%16]).
+typedef double d4v __attribute__((ext_vector_type(4)));
+
+// Largest value over the 64 lanes (all active); every lane gets it.
+template <int CTRL>
+__device__ __forceinline__ uint32_t max_dpp(uint32_t v) {
+  // old value 0 (the identity of max): foldable into one DPP max instruction
+  return max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, true));
+}
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+  v = max_dpp<0xB1>(v);
+  v = max_dpp<0x4E>(v);
+  v = max_dpp<0x141>(v);
+  v = max_dpp<0x140>(v);
+  const uint32_t a = __builtin_amdgcn_readlane(v, 0), b = __builtin_amdgcn_readlane(v, 16);
+  const uint32_t c = __builtin_amdgcn_readlane(v, 32), e = __builtin_amdgcn_readlane(v, 48);
+  return max(max(a, b), max(c, e));
+}
+
+// Orders LDS accesses of one wave across its lanes (a compiler barrier; the
+// LDS itself serves one wave's accesses in order).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <typename TS, int NP>
-__global__ void __launch_bounds__(2 * NP)
-gj_rows_kernel(SlaterArgs A, const int *walkers, int num) {
-  constexpr int KB = 16, NB = NP / KB, H = NP / 2, NT = 2 * NP, NWV = NT / 64, TR = 32;
+__global__ void __launch_bounds__(256, 2)
+gj_mfma_kernel(SlaterArgs A, const int *walkers, int num) {
+  constexpr int NT = NP / 16, NWV = 4, CT = NT / NWV, RPL = NP / 64, TILE = 256, MW = NP / 32;
   const int D = det_count(A), d = blockIdx.x % D;
   const int b = blockIdx.x / D;
   if (b >= num) return;
   const int w0 = walkers ? walkers[b] : b;
   const int w = w0 * D + d;  // this determinant's record
   const int n = A.n, nact = act_count(A);
-  const int t = threadIdx.x, i = t >> 1, h = t & 1;
-  // pivot rows and arg-max slots alternate between two buffers by pivot
-  // parity: pivot k+1 writes the buffer pivot k is not reading, and pivot k+2
-  // rewrites pivot k's buffer only after every wave has passed both barriers
-  // of pivot k+1, so one barrier per pivot is saved
-  __shared__ double prow_b[2][NP], krow_b[2][NP];
-  __shared__ double wmax_b[2][NWV];
-  __shared__ int widx_b[2][NWV];
-  __shared__ int ipiv[NP], dest[NP], cur[NP];
-  __shared__ double tile[TR][NP + 1];
+  const int t = threadIdx.x, lane = t & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(t >> 6);  // wave-uniform: scalar branches
+  const int lr = lane & 15, lq = lane >> 4;
 
-  const double *dv = A.dval + (long)w0 * nact * n;  // value matrix, (o, e)
-  double a[H];
+  // [0, 2) XK fragments by panel parity, [2, 3) pivot-row fragments, [3, 4)
+  // dummy slots; after the last panel, each wave's staging area for its part
+  // of the output
+  __shared__ double lds[4 * NT * TILE];
+  __shared__ int pivrow[NP];        // pivot row of column k
+  __shared__ int colofp[NP];        // column pivoted on row 16rt + 4i + q, at 16rt + 4q + i
+  __shared__ uint32_t s_pm[MW];     // rows pivoted so far
+  __shared__ double s_mant[NWV], s_sgn[NWV];
+  __shared__ int s_expo[NWV], s_par[NWV];
+  __shared__ int s_sing;
+  // fragment offset of XK[row][col] (col < 16); pivot row slot k, column j
+  // sits at fidx(j, k)
+  auto fidx = [](int row, int col) {
+    return (row >> 4) * TILE + (col >> 2) * 64 + (col & 3) * 16 + (row & 15);
+  };
+  double *fp = lds + 2 * NT * TILE;
+
+  if (t < MW) s_pm[t] = 0u;
+  if (t == 0) s_sing = 0;
+  const double *dv = A.dval + (long)w0 * nact * n;
+  d4v acc[NT][CT];
 #pragma unroll
-  for (int j = 0; j < H; j++) {
-    const int col = h * H + j;
-    a[j] = (i < n && col < n) ? ld(dv + (long)occ_row(A, d, col) * n + i)
-                              : (i == col ? 1.0 : 0.0);
-  }
-  double mant = 1.0, sgn = 1.0;
-  int expo = 0;
-  bool sing = false;
-  for (int kb = 0; kb < NB; kb++) {
+  for (int rt = 0; rt < NT; rt++) {
 #pragma unroll
-    for (int kk = 0; kk < KB; kk++) {
-      const int k = kb * KB + kk;
-      if (k < n && !sing) {
-        double *prow = prow_b[k & 1], *krow = krow_b[k & 1], *wmax = wmax_b[k & 1];
-        int *widx = widx_b[k & 1];
-        const double akk = dpp_f64<0xA0>(a[kk]);  // position kk of half 0
-        double best = (h == 0 && i >= k && i < n) ? fabs(akk) : -1.0;
-        int bi = i;
-        wave_argmax(best, bi);
-        if ((t & 63) == 0) { wmax[t >> 6] = best; widx[t >> 6] = bi; }
-        __syncthreads();
-        best = wmax[0];
-        bi = widx[0];
+    for (int c = 0; c < CT; c++) {
+      const int col = 16 * (wv + NWV * c) + lr;
 #pragma unroll
-        for (int q = 1; q < NWV; q++)
-          if (wmax[q] > best || (wmax[q] == best && widx[q] < bi)) { best = wmax[q]; bi = widx[q]; }
-        if (!(best > 0.0)) {
-          sing = true;
-        } else {
-          const int p = bi;
-          if (i == p) {
-#pragma unroll
-            for (int j = 0; j < H; j++) prow[h * H + j] = a[j];
-          }
-          if (i == k && p != k) {
-#pragma unroll
-            for (int j = 0; j < H; j++) krow[h * H + j] = a[j];
-          }
-          if (t == 0) ipiv[k] = p;
-          __syncthreads();
-          if (p != k) {
-            if (i == k) {
-#pragma unroll
-              for (int j = 0; j < H; j++) a[j] = prow[h * H + j];
-            } else if (i == p) {
-#pragma unroll
-              for (int j = 0; j < H; j++) a[j] = krow[h * H + j];
-            }
-            sgn = -sgn;
-          }
-          const double piv = prow[kk];
-          if (piv < 0) sgn = -sgn;
-          int e2;
-          mant = frexp(mant * fabs(piv), &e2);
-          expo += e2;
-          const double ip = 1.0 / piv;
-          const double f = dpp_f64<0xA0>(a[kk]);  // after the interchange
-          if (i == k) {
-#pragma unroll
-            for (int j = 0; j < H; j++) a[j] = (h == 0 && j == kk) ? ip : prow[h * H + j] * ip;
-          } else {
-            const double fi = f * ip;
-#pragma unroll
-            for (int j = 0; j < H; j++)
-              a[j] = (h == 0 && j == kk) ? -fi : a[j] - fi * prow[h * H + j];
-          }
-        }
+      for (int i = 0; i < 4; i++) {
+        const int r = 16 * rt + 4 * i + lq;
+        const double v = ld(dv + (long)occ_row(A, d, min(r, n - 1)) * n + min(col, n - 1));
+        acc[rt][c][i] = (r < n && col < n) ? v : (r == col ? 1.0 : 0.0);
       }
-    }
-    // rotate every row left by KB positions: half 0 takes its own [KB, H)
-    // plus half 1's [0, KB); half 1 takes its own [KB, H) plus half 0's [0, KB)
-    double tq[KB];
-#pragma unroll
-    for (int q = 0; q < KB; q++) tq[q] = dpp_f64<0xB1>(a[q]);  // partner lane (lane ^ 1)
-#pragma unroll
-    for (int j = 0; j < H - KB; j++) a[j] = a[j + KB];
-#pragma unroll
-    for (int q = 0; q < KB; q++) a[H - KB + q] = tq[q];
-  }
-  if (t == 0) {
-    for (int c = 0; c < n; c++) dest[c] = c;
-    if (!sing) {
-      for (int c = 0; c < n; c++) cur[c] = c;
-      for (int k = n - 1; k >= 0; k--) {
-        const int q = ipiv[k];
-        if (q != k) { const int tt = cur[k]; cur[k] = cur[q]; cur[q] = tt; }
-      }
-      for (int c = 0; c < n; c++) dest[cur[c]] = c;
     }
   }
   __syncthreads();
-  // row o = i of A^-1; register position j of half h is column h*H + j,
-  // which belongs to electron dest[h*H + j]
-  double *inv = A.inv + (long)w * n * n;
-  if (!sing) {
-    for (int r0 = 0; r0 < n; r0 += TR) {
-      if (i >= r0 && i < r0 + TR) {
+
+  double mant = 1.0, sgn = 1.0;  // this wave's pivots: prod |pivot| = mant 2^expo
+  int expo = 0, par = 0;         // par: parity of this wave's inversion count
+  uint32_t pm[MW];               // rows pivoted before the current panel (uniform)
 #pragma unroll
-        for (int j = 0; j < H; j++) {
-          const int col = h * H + j;
-          if (col < n) tile[i - r0][dest[col]] = a[j];
+  for (int q = 0; q < MW; q++) pm[q] = 0u;
+  bool sing = false;
+  // panel K = NWV c + v is column tile K: wave v, its slot c (compile time)
+#pragma unroll
+  for (int c = 0; c < CT; c++) {
+    for (int v = 0; v < NWV; v++) {
+      const int K = NWV * c + v, K0 = 16 * K;
+      if (sing || K0 >= n) break;
+      double *fx = lds + (K & 1) * NT * TILE;
+      if (wv == v) {
+        // ---- 1. the panel's 16 elimination steps, one or two rows per lane
+#pragma unroll
+        for (int rt = 0; rt < NT; rt++)
+#pragma unroll
+          for (int i = 0; i < 4; i++) fx[fidx(16 * rt + 4 * i + lq, lr)] = acc[rt][c][i];
+        wave_lds_sync();
+        double x[RPL][16], dsc[RPL];
+        bool live[RPL];  // row may still become a pivot
+#pragma unroll
+        for (int s = 0; s < RPL; s++) {
+          const int r = lane + 64 * s;
+#pragma unroll
+          for (int j = 0; j < 16; j++) x[s][j] = fx[fidx(r, j)];
+          const uint32_t m = (lane >> 5) ? pm[2 * s + 1] : pm[2 * s];
+          live[s] = !((m >> (lane & 31)) & 1u);
+          dsc[s] = 1.0;
+        }
+        wave_lds_sync();
+        bool psing = false;
+        int pv = 0;  // lane kk: pivot row of column K0 + kk
+#pragma unroll
+        for (int kk = 0; kk < 16; kk++) {
+          // pivot: the lowest row index among the live rows whose |x[.][kk]|
+          // has the largest upper word (sign, exponent, 20 mantissa bits):
+          // within 2^-20 of the largest |x|; an upper word of 0 (below
+          // 2^-1022) counts as zero, i.e. singular
+          uint32_t key[RPL], best = 0u;
+#pragma unroll
+          for (int s = 0; s < RPL; s++) {
+            const uint32_t hi = (uint32_t)(__builtin_bit_cast(uint64_t, x[s][kk]) >> 32) & 0x7fffffffu;
+            key[s] = live[s] ? hi + 1u : 0u;
+            best = max(best, key[s]);
+          }
+          best = wave_max_u32(best);
+          psing = psing || best <= 1u;
+          int p = 0;
+#pragma unroll
+          for (int s = RPL - 1; s >= 0; s--) {
+            const unsigned long long hit = __ballot(key[s] == best);
+            p = hit ? 64 * s + (int)__builtin_ctzll(hit) : p;
+          }
+          // the pivot row into scalar registers by readlane; a scalar branch
+          // picks the row half (the empty asm keeps the compiler from reading
+          // both halves and selecting)
+          const int pl = p & 63;
+          double prow[16];
+          if (RPL > 1 && p >= 64) {
+            asm volatile("");
+#pragma unroll
+            for (int j = 0; j < 16; j++) prow[j] = readlane_f64(x[RPL - 1][j], pl);
+          } else {
+            asm volatile("");
+#pragma unroll
+            for (int j = 0; j < 16; j++) prow[j] = readlane_f64(x[0][j], pl);
+          }
+          const double piv = prow[kk];
+          sgn = piv < 0 ? -sgn : sgn;
+          {
+            int e2;
+            mant = frexp(mant * fabs(piv), &e2);
+            expo += e2;
+          }
+          // 1/piv: reciprocal estimate and two Newton steps
+          double ip = __builtin_amdgcn_rcp(piv);
+          ip = fma(ip, fma(-piv, ip, 1.0), ip);
+          ip = fma(ip, fma(-piv, ip, 1.0), ip);
+          double fi[RPL];
+#pragma unroll
+          for (int s = 0; s < RPL; s++) fi[s] = lane + 64 * s == p ? 0.0 : x[s][kk] * ip;
+#pragma unroll
+          for (int j = 0; j < 16; j++) {
+            if (j == kk) continue;
+#pragma unroll
+            for (int s = 0; s < RPL; s++) x[s][j] -= fi[s] * prow[j];
+          }
+#pragma unroll
+          for (int s = 0; s < RPL; s++) {
+            const bool isp = lane + 64 * s == p;
+            x[s][kk] = isp ? 1.0 : -fi[s];
+            dsc[s] = isp ? ip : dsc[s];
+            live[s] = live[s] && !isp;
+          }
+          pv = lane == kk ? p : pv;
+        }
+        // pending scale of the pivot rows, then XK: the MFMA A operand and
+        // this wave's panel tile
+#pragma unroll
+        for (int s = 0; s < RPL; s++)
+#pragma unroll
+          for (int kk = 0; kk < 16; kk++) fx[fidx(lane + 64 * s, kk)] = x[s][kk] * dsc[s];
+        // inversions of k -> p_k: rows pivoted earlier that lie above p_k
+        {
+          const int pw = pv >> 5, pb = pv & 31;
+          int cnt = 0;
+#pragma unroll
+          for (int q = 0; q < MW; q++)
+            cnt += q > pw ? __builtin_popcount(pm[q]) : (q == pw ? __builtin_popcount(pm[q] >> pb) : 0);
+          uint32_t nm[MW];
+#pragma unroll
+          for (int q = 0; q < MW; q++) nm[q] = pm[q];
+#pragma unroll
+          for (int k2 = 0; k2 < 16; k2++) {
+            const int p2 = __builtin_amdgcn_readlane(pv, k2);
+            cnt += (k2 < lane && p2 > pv) ? 1 : 0;
+#pragma unroll
+            for (int q = 0; q < MW; q++) nm[q] |= (p2 >> 5) == q ? 1u << (p2 & 31) : 0u;
+          }
+          par ^= __popcll(__ballot(lane < 16 && (cnt & 1))) & 1;
+          if (lane < MW) {
+            uint32_t m = nm[0];
+#pragma unroll
+            for (int q = 1; q < MW; q++) m = lane == q ? nm[q] : m;
+            s_pm[lane] = m;
+          }
+        }
+        if (lane < 16) {
+          pivrow[K0 + lane] = pv;
+          colofp[(pv & ~15) | ((pv & 3) << 2) | ((pv >> 2) & 3)] = K0 + lane;
+        }
+        if (lane == 0 && psing) s_sing = 1;
+        wave_lds_sync();
+#pragma unroll
+        for (int rt = 0; rt < NT; rt++)
+#pragma unroll
+          for (int i = 0; i < 4; i++) acc[rt][c][i] = fx[fidx(16 * rt + 4 * i + lq, lr)];
+      }
+      __syncthreads();
+      if (s_sing) { sing = true; break; }
+      // ---- 2. pivot rows out of the other columns (zeroed there)
+      uint32_t nm[MW];  // this panel's pivot rows
+#pragma unroll
+      for (int q = 0; q < MW; q++) {
+        const uint32_t m = __builtin_amdgcn_readfirstlane(s_pm[q]);
+        nm[q] = m & ~pm[q];
+        pm[q] = m;
+      }
+      // LDS writes only for the groups of four rows that hold a pivot (lanes
+      // on other rows of such a group write to a dummy slot of their own);
+      // the zeroing is a select over every element, outside any branch, so
+      // that the accumulator tiles are never modified under control flow
+#pragma unroll
+      for (int rt = 0; rt < NT; rt++) {
+        const uint32_t half = (nm[rt >> 1] >> (16 * (rt & 1))) & 0xFFFFu;
+        if (!half) continue;
+        const int4 cof = *reinterpret_cast<const int4 *>(colofp + 16 * rt + 4 * lq);
+        const int cv[4] = {cof.x, cof.y, cof.z, cof.w};
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const uint32_t nib = (half >> (4 * i)) & 0xFu;
+          if (!nib) continue;
+          const bool mine = (nib >> lq) & 1u;
+          const int k = mine ? cv[i] - K0 : 0;  // colofp is valid on the pivot rows only
+#pragma unroll
+          for (int c2 = 0; c2 < CT; c2++) {
+            const int ct = wv + NWV * c2;
+            if (ct == K || 16 * ct >= n) continue;
+            lds[mine ? 2 * NT * TILE + fidx(16 * ct + lr, k) : 3 * NT * TILE + lane] = acc[rt][c2][i];
+          }
+        }
+      }
+#pragma unroll
+      for (int c2 = 0; c2 < CT; c2++) {
+        const bool other = wv + NWV * c2 != K;
+#pragma unroll
+        for (int rt = 0; rt < NT; rt++) {
+          const uint32_t bits = (nm[rt >> 1] >> (16 * (rt & 1) + lq)) & 0x1111u;
+#pragma unroll
+          for (int i = 0; i < 4; i++)
+            acc[rt][c2][i] = (other && ((bits >> (4 * i)) & 1u)) ? 0.0 : acc[rt][c2][i];
         }
       }
       __syncthreads();
-      for (int idx = t; idx < TR * n; idx += NT) {
-        const int r = idx / n, e = idx % n;
-        if (r0 + r < n) inv[(long)(r0 + r) * n + e] = tile[r][e];
+      // ---- 3. rank-16 update of every other column
+      double bf[CT][4];
+#pragma unroll
+      for (int c2 = 0; c2 < CT; c2++)
+#pragma unroll
+        for (int s = 0; s < 4; s++) bf[c2][s] = fp[(wv + NWV * c2) * TILE + s * 64 + lane];
+#pragma unroll
+      for (int s = 0; s < 4; s++) {
+#pragma unroll
+        for (int rt = 0; rt < NT; rt++) {
+          if (16 * rt >= n) continue;
+          const double af = fx[rt * TILE + s * 64 + lane];
+#pragma unroll
+          for (int c2 = 0; c2 < CT; c2++) {
+            const int ct = wv + NWV * c2;
+            if (ct != K && 16 * ct < n)
+              acc[rt][c2] = __builtin_amdgcn_mfma_f64_16x16x4f64(af, bf[c2][s], acc[rt][c2], 0, 0, 0);
+          }
+        }
       }
-      __syncthreads();
+    }
+  }
+  // ---- determinant: the waves' pivot products and inversion parities
+  if (lane == 0) { s_mant[wv] = mant; s_expo[wv] = expo; s_sgn[wv] = sgn; s_par[wv] = par; }
+  __syncthreads();
+  if (!sing) {
+    // ---- inverse out: inv(o, e) = X[p_e][c(o)]; for column c of X, o = p_c.
+    // Each wave stages its 16-column tiles transposed (rows swizzled) in its
+    // own LDS area and writes the rows o of A^-1, coalesced.
+    int pe[RPL];
+#pragma unroll
+    for (int s = 0; s < RPL; s++) pe[s] = pivrow[min(lane + 64 * s, n - 1)];
+    double *inv = A.inv + (long)w * n * n;
+    double *st = lds + wv * 16 * NP;
+#pragma unroll
+    for (int q = 0; q < CT; q++) {
+      const int c0 = 16 * (wv + NWV * q);
+      if (c0 >= n) break;
+      if (q > 0) wave_lds_sync();
+#pragma unroll
+      for (int rt = 0; rt < NT; rt++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const int r = 16 * rt + 4 * i + lq;
+          st[lr * NP + (r ^ (lr << 2))] = acc[rt][q][i];
+        }
+      wave_lds_sync();
+      const int4 *pv4 = reinterpret_cast<const int4 *>(pivrow + c0);
+      int ov[16];
+#pragma unroll
+      for (int m4 = 0; m4 < 4; m4++) {
+        const int4 v = pv4[m4];
+        ov[4 * m4] = v.x; ov[4 * m4 + 1] = v.y; ov[4 * m4 + 2] = v.z; ov[4 * m4 + 3] = v.w;
+      }
+#pragma unroll
+      for (int m = 0; m < 16; m++) {
+        if (c0 + m < n) {
+#pragma unroll
+          for (int s = 0; s < RPL; s++) {
+            const int e = lane + 64 * s;
+            if (e < n) inv[(long)ov[m] * n + e] = st[m * NP + (pe[s] ^ (m << 2))];
+          }
+        }
+      }
     }
   }
   if (t == 0) {
-    A.logdet[w] = sing ? 0.0 : log(mant) + expo * 0.6931471805599453;
-    A.sgn[w] = sing ? 0.0 : sgn;
+    double m = 1.0, sg = 1.0;
+    int ex = 0, pr = 0;
+    for (int q = 0; q < NWV; q++) {
+      int e2;
+      m = frexp(m * s_mant[q], &e2);
+      ex += e2 + s_expo[q];
+      sg *= s_sgn[q];
+      pr ^= s_par[q];
+    }
+    A.logdet[w] = sing ? 0.0 : log(m) + ex * 0.6931471805599453;
+    A.sgn[w] = sing ? 0.0 : (pr ? -sg : sg);
     A.singular[w] = sing ? 1 : 0;
     A.dcount[w] = 0;
   }
@@ -350,9 +570,9 @@ hipError_t launch_lu_inverse(const SlaterArgs &a, const int *walkers, int num, h
   else if (a.n <= 32)
     hipLaunchKernelGGL((lu_inverse_kernel<TS, 2, 16>), dim3(g), dim3(256), 0, st, a, walkers, num);
   else if (a.n <= 64)
-    hipLaunchKernelGGL((gj_rows_kernel<TS, 64>), dim3(g), dim3(128), 0, st, a, walkers, num);
+    hipLaunchKernelGGL((gj_mfma_kernel<TS, 64>), dim3(g), dim3(256), 0, st, a, walkers, num);
   else if (a.n <= 128)
-    hipLaunchKernelGGL((gj_rows_kernel<TS, 128>), dim3(g), dim3(256), 0, st, a, walkers, num);
+    hipLaunchKernelGGL((gj_mfma_kernel<TS, 128>), dim3(g), dim3(256), 0, st, a, walkers, num);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();
