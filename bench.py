@@ -124,27 +124,53 @@ def run_path(s, precision, W, steps, warmup, tau, block, refresh, seed, groups=1
     }
 
 
-def accuracy_report(s, n_conf=10000, seed=4321, tau=0.01, sweeps=1, precision="fp32"):
-    """Per-quantity error of the mixed path against the FP64 path on n_conf
-    fixed configurations (FP64 VMC positions after `sweeps` sweeps from
-    N(R_A, 0.5^2)).  The FP64 path itself matches the reference evaluate to
-    <= 1e-10 on the golden configurations.  Both all-electron evaluates and a
-    single-electron proposal (moved electron 0, Sherman-Morrison on FP32 rows)
-    are compared."""
+def equilibrated_configs(s, n_conf=10000, seed=4321, tau=0.3, eq_sweeps=100, gap_sweeps=4,
+                         walkers=2048):
+    """n_conf configurations drawn from an equilibrated FP64 device VMC with
+    the reference's default proposal (Umrigar-93 modified drift for unequal
+    electrons, QMCFlags.cpp:77-78): `walkers` walkers from N(R_A, 0.5^2),
+    eq_sweeps sweeps of single-electron moves at tau (the mean local energy
+    and the acceptance are flat from about sweep 60 at tau = 0.3), then one
+    snapshot of every walker each gap_sweeps sweeps until n_conf are taken."""
     from qmcbeaver_amd import synth, Context
+    from qmcbeaver_amd._abi import QMCB_DRIFT_UMRIGAR93_UNEQUAL
     N = s.n_elec
-    R0 = synth.initial_walkers(s, n_conf, seed)
+    W = min(walkers, n_conf)
+    ids = np.arange(W)
+    snaps = []
+    with Context(s.pack(), precision="fp64", max_walkers=W, seed=seed,
+                 drift_mod=QMCB_DRIFT_UMRIGAR93_UNEQUAL, dt=tau) as c:
+        c.evaluate(ids, synth.initial_walkers(s, W, seed))
+        step = 0
+        c.vmc_steps(W, step, eq_sweeps * N, tau)
+        step += eq_sweeps * N
+        while sum(len(x) for x in snaps) < n_conf:
+            snaps.append(c.positions(W))
+            c.vmc_steps(W, step, gap_sweeps * N, tau)
+            step += gap_sweeps * N
+    R = np.concatenate(snaps)[:n_conf]
+    desc = (f"FP64 device VMC, umrigar93 modified drift, tau {tau}: {W} walkers after "
+            f"{eq_sweeps} sweeps, {len(snaps)} snapshots {gap_sweeps} sweeps apart")
+    return R, desc
+
+
+def accuracy_report(s, R, desc, precision="fp32"):
+    """Per-quantity error of a mixed mode against the FP64 path on the
+    configurations R (equilibrated_configs), all-electron evaluates and a
+    single-electron proposal (moved electron 0, Sherman-Morrison on the
+    mode's stored rows); and against the values the reference program wrote
+    for the four golden configurations of the 256-electron molecule.  The
+    FP64 path matches the reference evaluate to <= 1e-10."""
+    from qmcbeaver_amd import Context
+    N = s.n_elec
+    n_conf = len(R)
     ids = np.arange(n_conf)
-    with Context(s.pack(), precision="fp64", max_walkers=n_conf, seed=seed) as c64:
-        c64.evaluate(ids, R0)
-        if sweeps:
-            c64.vmc_steps(n_conf, 0, sweeps * N, tau)
-        R = c64.positions(n_conf)
+    Rp = R.copy()
+    Rp[:, 0, :] += 0.05
+    with Context(s.pack(), precision="fp64", max_walkers=n_conf) as c64:
         a64 = c64.evaluate(ids, R)
-        Rp = R.copy()
-        Rp[:, 0, :] += 0.05
         p64 = c64.evaluate(ids, Rp, whichE=0)
-    with Context(s.pack(), precision=precision, max_walkers=n_conf, seed=seed) as c32:
+    with Context(s.pack(), precision=precision, max_walkers=n_conf) as c32:
         a32 = c32.evaluate(ids, R)
         p32 = c32.evaluate(ids, Rp, whichE=0)
     ok = ~(a64["singular"].astype(bool) | a32["singular"].astype(bool) |
@@ -154,8 +180,10 @@ def accuracy_report(s, n_conf=10000, seed=4321, tau=0.01, sweeps=1, precision="f
         d = np.abs(np.asarray(x)[ok] - np.asarray(y)[ok]).reshape(ok.sum(), -1)
         d = d.max(axis=1) / per
         return {"max": float(d.max()), "rms": float(np.sqrt(np.mean(d * d))),
-                "p99": float(np.percentile(d, 99)), "median": float(np.median(d))}
-    out = {"configurations": int(ok.sum()), "reference": "FP64 path", "precision": precision,
+                "p99": float(np.percentile(d, 99)), "median": float(np.median(d)),
+                "above_1e-6": float(np.mean(d > 1e-6))}
+    out = {"configurations": int(ok.sum()), "sampler": desc, "reference": "FP64 path",
+           "precision": precision,
            "all_electron": {"log_psi": stats(a32["log_psi"], a64["log_psi"]),
                             "grad_lnpsi": stats(a32["grad_lnpsi"], a64["grad_lnpsi"]),
                             "e_local_per_electron": stats(a32["e_local"], a64["e_local"], N)},
@@ -163,6 +191,16 @@ def accuracy_report(s, n_conf=10000, seed=4321, tau=0.01, sweeps=1, precision="f
                                "grad_lnpsi": stats(p32["grad_lnpsi"], p64["grad_lnpsi"]),
                                "e_local_per_electron": stats(p32["e_local"], p64["e_local"], N)},
            "sign_mismatch": int(np.sum(a32["sign_psi"][ok] != a64["sign_psi"][ok]))}
+    gpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden",
+                         "cfg2_mol256.npz")
+    if os.path.exists(gpath):
+        g = np.load(gpath)
+        with Context(s.pack(), precision=precision, max_walkers=len(g["R"])) as c:
+            r = c.evaluate(np.arange(len(g["R"])), g["R"])
+        out["reference_goldens"] = {
+            "configurations": int(len(g["R"])),
+            "e_local_per_electron_max": float(np.max(np.abs(r["e_local"] - g["ev_e_local"])) / N),
+            "log_psi_max": float(np.max(np.abs(r["log_psi"] - g["ev_log_psi"])))}
     return out
 
 
@@ -496,14 +534,12 @@ def main():
             "kernels_ms_per_step": {k: v["ms"] / max(args.steps, 1)
                                     for k, v in bf16["kernels"].items()}}
     if args.accuracy and args.precision != "fp64":
-        out["mixed_error"] = accuracy_report(s, args.accuracy, tau=args.tau,
-                                             precision=args.precision)
+        R_acc, acc_desc = equilibrated_configs(s, args.accuracy)
+        out["mixed_error"] = accuracy_report(s, R_acc, acc_desc, precision=args.precision)
         if store is not None:
-            out["fp32store"]["error"] = accuracy_report(s, args.accuracy, tau=args.tau,
-                                                        precision="fp32store")
+            out["fp32store"]["error"] = accuracy_report(s, R_acc, acc_desc, precision="fp32store")
         if bf16 is not None:
-            out["bf16"]["error"] = accuracy_report(s, args.accuracy, tau=args.tau,
-                                                   precision="bf16")
+            out["bf16"]["error"] = accuracy_report(s, R_acc, acc_desc, precision="bf16")
     if args.serving_jobs > 0:
         srv = serving_run(args.precision, args.serving_jobs, args.rate, 20, args.serving_workers,
                           args.capacity_gib)
