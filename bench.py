@@ -356,7 +356,8 @@ def main():
     ap.add_argument("--groups", type=int, default=3,
                     help="walker groups on overlapping streams (results are bit-identical); "
                          "-1: chosen by the library's timing")
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp32store", "bf16", "fp64"])
+    ap.add_argument("--precision", default="fp32",
+                    choices=["fp32", "fp32store", "bf16", "bf16store", "fp64"])
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--accuracy", type=int, default=10000,
@@ -447,6 +448,7 @@ def main():
     fp64 = None
     store = None
     bf16 = None
+    bf16store = None
     if not args.no_fp64 and args.precision != "fp64":
         fp64 = run_path(s, "fp64", args.walkers, args.steps, args.warmup, args.tau, args.block,
                         0, 1234, args.groups)
@@ -455,8 +457,10 @@ def main():
                              args.block, args.refresh, 1234, args.groups)
             bf16 = run_path(s, "bf16", args.walkers, args.steps, args.warmup, args.tau,
                             args.block, args.refresh, 1234, args.groups)
+            bf16store = run_path(s, "bf16store", args.walkers, args.steps, args.warmup,
+                                 args.tau, args.block, args.refresh, 1234, args.groups)
 
-    s_d = 8 if args.precision == "fp64" else 4
+    s_d = {"fp64": 8, "fp32store": 4, "bf16store": 2}.get(args.precision, 4)
     kt = mixed["kernels"].get("sm_move", {"ms": float("nan"), "launches": 1})
     avg_ms = kt["ms"] / max(kt["launches"], 1)
     bytes_launch = args.walkers * sm_bytes_per_move(n, N, s_d, s_d, mixed["acceptance"])
@@ -533,6 +537,16 @@ def main():
             bf16["walker_moves_per_s"] / fp64["walker_moves_per_s"],
             "kernels_ms_per_step": {k: v["ms"] / max(args.steps, 1)
                                     for k, v in bf16["kernels"].items()}}
+    if bf16store is not None:
+        out["bf16store"] = {
+            "description": "FP64 basis rounded to FP32, v_mfma_f32_16x16x4f32 GEMM, derivative "
+                           "matrices stored in BF16 (8-bit mantissa: 16 n^2 bytes per move "
+                           "against 24 n^2), FP64 inverse/log-det/E_L",
+            "value": bf16store["walker_moves_per_s"], "ms_per_step": bf16store["ms_per_step"],
+            "over_fp64": None if fp64 is None else
+            bf16store["walker_moves_per_s"] / fp64["walker_moves_per_s"],
+            "kernels_ms_per_step": {k: v["ms"] / max(args.steps, 1)
+                                    for k, v in bf16store["kernels"].items()}}
     if args.accuracy and args.precision != "fp64":
         R_acc, acc_desc = equilibrated_configs(s, args.accuracy)
         out["mixed_error"] = accuracy_report(s, R_acc, acc_desc, precision=args.precision)
@@ -540,6 +554,8 @@ def main():
             out["fp32store"]["error"] = accuracy_report(s, R_acc, acc_desc, precision="fp32store")
         if bf16 is not None:
             out["bf16"]["error"] = accuracy_report(s, R_acc, acc_desc, precision="bf16")
+        if bf16store is not None:
+            out["bf16store"]["error"] = accuracy_report(s, R_acc, acc_desc, precision="bf16store")
     if args.serving_jobs > 0:
         srv = serving_run(args.precision, args.serving_jobs, args.rate, 20, args.serving_workers,
                           args.capacity_gib)

@@ -44,8 +44,11 @@ typedef enum qmcb_precision {
   QMCB_MIXED_BF16 = 2,  /* as MIXED_FP32 with BF16 GEMM operands (basis values
                            and coefficients rounded from FP64) on
                            v_mfma_f32_16x16x16_bf16, FP32 accumulate            */
-  QMCB_MIXED_F32STORE = 3 /* FP64 basis and GEMM, derivative matrices stored
+  QMCB_MIXED_F32STORE = 3, /* FP64 basis and GEMM, derivative matrices stored
                            in FP32 (the HBM-bound part halves)                    */
+  QMCB_MIXED_BF16STORE = 4 /* as MIXED_FP32 (FP32 MFMA GEMM), derivative matrices
+                           stored in BF16 (8-bit mantissa): a single-electron
+                           move streams 16 n^2 bytes instead of 24 n^2          */
 } qmcb_precision;
 
 /* Drift (quantum force) modification, QMCSCFJastrow.cpp:329-472. */
@@ -221,7 +224,7 @@ qmcb_status qmcb_evaluate_device(qmcb_ctx *ctx, int32_t num,
  * device array of 64 MiB per GPU (held until the process ends) instead of
  * the context's scratch; results equal qmcb_evaluate_device's.  Limit: one
  * walker's electrons of a spin must fit, 5 * ceil32(n_basis) * (8 FP64 and
- * fp32store, 4 FP32, 2 BF16) bytes each, else QMCB_INVALID_ARG.  Every call
+ * fp32store, 4 FP32 and bf16store, 2 BF16) bytes each, else QMCB_INVALID_ARG.  Every call
  * on one GPU shares that GPU's array: calls that overlap on one GPU (other
  * streams or threads, any context) race and must be serialised by the
  * caller.  Kernel timing must be off (QMCB_BAD_STATE otherwise).

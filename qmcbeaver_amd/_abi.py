@@ -8,10 +8,11 @@ import numpy as np
 
 PADE_MAX = 8
 
-QMCB_FP64, QMCB_MIXED_FP32, QMCB_MIXED_BF16, QMCB_MIXED_F32STORE = 0, 1, 2, 3
+QMCB_FP64, QMCB_MIXED_FP32, QMCB_MIXED_BF16, QMCB_MIXED_F32STORE, QMCB_MIXED_BF16STORE = \
+    0, 1, 2, 3, 4
 QMCB_DRIFT_NONE, QMCB_DRIFT_UMRIGAR93_UNEQUAL, QMCB_DRIFT_UMRIGAR93_EQUAL = 0, 1, 2
 PRECISIONS = {"fp64": QMCB_FP64, "fp32": QMCB_MIXED_FP32, "bf16": QMCB_MIXED_BF16,
-              "fp32store": QMCB_MIXED_F32STORE}
+              "fp32store": QMCB_MIXED_F32STORE, "bf16store": QMCB_MIXED_BF16STORE}
 STATUS = {0: "OK", 1: "INVALID_ARG", 2: "OOM", 3: "DEVICE_ERROR", 4: "BAD_STATE"}
 
 _dp = C.POINTER(C.c_double)

@@ -93,5 +93,7 @@ template hipError_t launch_cusp<double>(const CuspDev &, const PointSet &, const
                                         hipStream_t);
 template hipError_t launch_cusp<float>(const CuspDev &, const PointSet &, const GemmOut &,
                                        hipStream_t);
+template hipError_t launch_cusp<bf16s>(const CuspDev &, const PointSet &, const GemmOut &,
+                                       hipStream_t);
 
 }  // namespace qmcb

@@ -128,7 +128,8 @@ mdet_move_kernel(MdetArgs A, const int *walkers, int num) {
   const TO *vr = (const TO *)A.vrow + b * A.v_s_walker;
   for (int i = t; i < kComp * n; i += 128) {
     const int k = i / n, o = i % n;
-    v[k][o] = (double)vr[(long)k * A.v_s_comp + occ_row(S, d, o)];
+    const double x = (double)vr[(long)k * A.v_s_comp + occ_row(S, d, o)];
+    v[k][o] = k == 0 ? x : as_stored<TS>(x);
   }
   const double *inv = S.inv + w * nn2;
   for (int o = t; o < n; o += 128) colr[o] = inv[(long)o * n + r];
@@ -496,6 +497,7 @@ hipError_t launch_mdet_commit(const MdetArgs &a, const int *walkers, const uint8
 INST(double, double)
 INST(float, float)
 INST(float, double)
+INST(bf16s, float)
 #undef INST
 
 }  // namespace qmcb

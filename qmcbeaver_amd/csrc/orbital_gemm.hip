@@ -282,6 +282,9 @@ template hipError_t launch_orbital_gemm<float, float>(const float *, int, int, c
 template hipError_t launch_orbital_gemm<double, float>(const double *, int, int, const double *,
                                                        int, long, long, int, const GemmOut &,
                                                        hipStream_t, int);
+template hipError_t launch_orbital_gemm<float, bf16s>(const float *, int, int, const float *, int,
+                                                      long, long, int, const GemmOut &,
+                                                      hipStream_t, int);
 template hipError_t launch_orbital_gemm<__hip_bfloat16, float>(const __hip_bfloat16 *, int, int,
                                                                const __hip_bfloat16 *, int, long,
                                                                long, int, const GemmOut &,

@@ -598,13 +598,17 @@ typedef Engine<double, double, double, double> EngineF64;
 typedef Engine<float, float, double, float> EngineF32;       // FP64 basis, FP32 MFMA
 typedef Engine<float, double, double, double> EngineF32Store;  // FP64 GEMM, FP32 storage
 typedef Engine<float, __hip_bfloat16, double, float> EngineBF16;  // FP64 basis, BF16 MFMA
+typedef Engine<bf16s, float, double, float> EngineBF16Store;  // FP32 MFMA, BF16 storage
 
 #define DISPATCH(ctx, call)                                                          \
   ((ctx)->opt.precision == QMCB_FP64                                                 \
        ? EngineF64::call                                                             \
        : ((ctx)->opt.precision == QMCB_MIXED_F32STORE                                \
               ? EngineF32Store::call                                                 \
-              : ((ctx)->opt.precision == QMCB_MIXED_BF16 ? EngineBF16::call : EngineF32::call)))
+              : ((ctx)->opt.precision == QMCB_MIXED_BF16                             \
+                     ? EngineBF16::call                                              \
+                     : ((ctx)->opt.precision == QMCB_MIXED_BF16STORE ? EngineBF16Store::call \
+                                                                    : EngineF32::call))))
 
 qmcb_status check_ids(qmcb_ctx *ctx, int num, const int32_t *ids) {
   if (num < 0 || num > ctx->Wc) {
@@ -686,7 +690,7 @@ qmcb_status create_impl(const qmcb_system *s, const qmcb_options *o, qmcb_ctx *c
   ctx->Wc = o->max_walkers;
   const bool f64 = o->precision == QMCB_FP64;
   const bool store32 = o->precision == QMCB_MIXED_F32STORE;
-  ctx->ts = f64 ? 8 : 4;
+  ctx->ts = f64 ? 8 : (o->precision == QMCB_MIXED_BF16STORE ? 2 : 4);
   ctx->tx = ctx->to = (f64 || store32) ? 8 : 4;
   if (o->precision == QMCB_MIXED_BF16) ctx->tx = 2;  // BF16 GEMM operands, FP32 output
   QCHK(hipSetDevice(o->device));
@@ -1022,7 +1026,8 @@ size_t plan_device_bytes(const qmcb_system *s, const qmcb_options *o) {
   auto add = [&](size_t count, size_t elem) { total += arena_round(count * elem); };
   const bool f64 = o->precision == QMCB_FP64;
   const bool store32 = o->precision == QMCB_MIXED_F32STORE;
-  const size_t ts = f64 ? 8 : 4, to = (f64 || store32) ? 8 : 4;
+  const size_t ts = f64 ? 8 : (o->precision == QMCB_MIXED_BF16STORE ? 2 : 4);
+  const size_t to = (f64 || store32) ? 8 : 4;
   const size_t tx = o->precision == QMCB_MIXED_BF16 ? 2 : to;
   const long W = o->max_walkers, N = s->n_up + s->n_dn, nn = s->n_nuc, B = s->n_basis;
   const long kpad = ((B + 31) / 32) * 32;
@@ -1154,7 +1159,8 @@ qmcb_status qmcb_create(const qmcb_system *sys, const qmcb_options *opt, qmcb_ct
     return QMCB_INVALID_ARG;
   }
   if (opt->precision != QMCB_FP64 && opt->precision != QMCB_MIXED_FP32 &&
-      opt->precision != QMCB_MIXED_F32STORE && opt->precision != QMCB_MIXED_BF16) {
+      opt->precision != QMCB_MIXED_F32STORE && opt->precision != QMCB_MIXED_BF16 &&
+      opt->precision != QMCB_MIXED_BF16STORE) {
     g_create_error = "precision not supported by this library";
     return QMCB_INVALID_ARG;
   }

@@ -9,6 +9,37 @@
 
 namespace qmcb {
 
+// Derivative-matrix storage in BF16: the upper 16 bits of an FP32 (8-bit
+// mantissa, the FP32 exponent range), rounded to nearest even on the way in
+// and widened exactly on the way out; arithmetic is never done in it.
+struct bf16s {
+  uint16_t b;
+  bf16s() = default;
+  __host__ __device__ bf16s(float f) {
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) {
+      b = (uint16_t)((u >> 16) | 0x40u);  // NaN stays a quiet NaN
+    } else {
+      u += 0x7fffu + ((u >> 16) & 1u);
+      b = (uint16_t)(u >> 16);
+    }
+  }
+  __host__ __device__ bf16s(double d) : bf16s((float)d) {}
+  __host__ __device__ operator double() const {
+    return (double)__builtin_bit_cast(float, (uint32_t)b << 16);
+  }
+};
+static_assert(sizeof(bf16s) == 2, "bf16s is two bytes");
+
+// A proposed derivative value as the storage type will hold it: with BF16
+// storage the moved electron's new row enters the traces rounded, as every
+// other electron's stored row does (so a resident state equals a fresh
+// evaluate); the wider storage types take the GEMM output as it is.
+template <typename TS>
+__device__ __forceinline__ double as_stored(double x) { return x; }
+template <>
+__device__ __forceinline__ double as_stored<bf16s>(double x) { return (double)bf16s(x); }
+
 // A batch of evaluation points: point p lives at
 //   base + (p / per_walker) * walker_stride + (p % per_walker) * point_stride
 //   (with walker_map: base + walker_map[p / per_walker] * walker_stride + ...)

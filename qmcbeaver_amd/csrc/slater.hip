@@ -666,6 +666,7 @@ hipError_t launch_traces_all(const SlaterArgs &a, const int *walkers, int num, h
 #ifndef QMCB_SM_UNROLL_SMALL
 #define QMCB_SM_UNROLL_SMALL 2
 #endif
+// BF16 derivatives (16 bytes per row and lane) take the FP64 depth.
 template <typename TS, int NC>
 __host__ __device__ constexpr int sm_unroll() {
   return NC < 128 ? QMCB_SM_UNROLL_SMALL : (sizeof(TS) == 4 ? QMCB_SM_UNROLL_F32 : QMCB_SM_UNROLL_F64);
@@ -703,6 +704,16 @@ __device__ __forceinline__ T ld_stream(const T *p) {
   return *p;
 #endif
 }
+template <>
+__device__ __forceinline__ bf16s ld_stream<bf16s>(const bf16s *p) {
+  bf16s v;
+#if QMCB_SM_NT
+  v.b = __builtin_nontemporal_load(reinterpret_cast<const uint16_t *>(p));
+#else
+  v.b = p->b;
+#endif
+  return v;
+}
 
 template <typename TS, typename TO, int NC, int NT>
 __global__ void __launch_bounds__(NT, QMCB_SM_MINWAVES)
@@ -737,7 +748,8 @@ sm_move_kernel(MoveArgs A, const int *walkers, int num) {
   const double *zbw = A.s.zbuf + (long)w * KD * n;
   for (int i = t; i < kComp * n; i += NT) {
     const int k = i / n, o = i % n;
-    v[k][o] = (double)vr[(long)k * A.v_s_comp + o];
+    const double x = (double)vr[(long)k * A.v_s_comp + o];
+    v[k][o] = k == 0 ? x : as_stored<TS>(x);
   }
   for (int i = t; i < KD * NC; i += NT) {
     const int a = i / NC, o = i % NC;
@@ -1260,10 +1272,13 @@ hipError_t launch_commit(const MoveArgs &a, const int *walkers, const uint8_t *m
 INST(double, double)
 INST(float, float)
 INST(float, double)
+INST(bf16s, float)
 #undef INST
 template hipError_t launch_lu_inverse<double>(const SlaterArgs &, const int *, int, hipStream_t);
 template hipError_t launch_lu_inverse<float>(const SlaterArgs &, const int *, int, hipStream_t);
 template hipError_t launch_traces_all<double>(const SlaterArgs &, const int *, int, hipStream_t);
 template hipError_t launch_traces_all<float>(const SlaterArgs &, const int *, int, hipStream_t);
+template hipError_t launch_lu_inverse<bf16s>(const SlaterArgs &, const int *, int, hipStream_t);
+template hipError_t launch_traces_all<bf16s>(const SlaterArgs &, const int *, int, hipStream_t);
 
 }  // namespace qmcb

@@ -17,6 +17,9 @@ directly against the values the reference program wrote.
   configurations; asserted with a factor of ~5 margin.
 * bf16 (BF16 MFMA operands, FP32 accumulation): median ~1e-3 Ha/e, p99
   ~1e-2 Ha/e; a throughput mode, asserted at its measured envelope.
+* bf16store (FP32 MFMA GEMM, derivative matrices stored in BF16): median
+  ~1e-4 Ha/e, p99 ~4e-4; the mode that streams 16 n^2 bytes per move
+  instead of 24 n^2, asserted at its measured envelope.
 """
 import os
 
@@ -121,3 +124,15 @@ def test_bf16_measured_envelope(equilibrated):
     assert e["sign_mismatch"] <= W // 100
     de, dlog, _ = golden_errors("bf16")
     assert np.isfinite(de).all() and de.max() < 10.0
+
+
+def test_bf16store_measured_envelope(equilibrated):
+    e = errors(equilibrated, "bf16store")
+    assert e["ok"] >= W - 8
+    assert np.median(e["de"]) < 3e-4 and np.percentile(e["de"], 99) < 2e-3
+    assert np.median(e["dpe"]) < 3e-4 and np.percentile(e["dpe"], 99) < 2e-3
+    assert e["de"].max() < 3e-2 and e["dpe"].max() < 3e-2
+    assert e["dlog"].max() < 2e-3
+    assert e["sign_mismatch"] == 0
+    de, dlog, sign_ok = golden_errors("bf16store")
+    assert de.max() < 1e-2 and dlog.max() < 1e-5 and sign_ok

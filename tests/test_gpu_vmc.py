@@ -31,7 +31,7 @@ def say(*a):
 
 
 @pytest.mark.parametrize("precision,refresh,tol", [("fp64", 0, 1e-8), ("fp32", 64, 1e-6),
-                                                    ("bf16", 64, 1e-6)])
+                                                    ("bf16", 64, 1e-6), ("bf16store", 64, 1e-6)])
 def test_vmc_resident_state_matches_fresh_evaluate(precision, refresh, tol):
     s = synth.make_system(1)
     W, steps = 64, 300
@@ -60,7 +60,8 @@ def test_vmc_resident_state_matches_fresh_evaluate(precision, refresh, tol):
 
 
 @pytest.mark.parametrize("n_elec", [152, 200], ids=["n76_tile96", "n100_tile128"])
-@pytest.mark.parametrize("precision,refresh,tol", [("fp64", 0, 1e-8), ("fp32", 64, 1e-6)])
+@pytest.mark.parametrize("precision,refresh,tol", [("fp64", 0, 1e-8), ("fp32", 64, 1e-6),
+                                                    ("bf16store", 64, 1e-6)])
 def test_vmc_resident_state_column_tiles(n_elec, precision, refresh, tol):
     # the same check on the 96- and 128-column Sherman-Morrison tiles
     s = synth.make_system(-1, seed=5, n_elec=n_elec)
