@@ -28,6 +28,7 @@ in the mix, and the serving loop's admission with real contexts.
 """
 import os
 import tempfile
+import time
 import gzip
 
 import numpy as np
@@ -40,6 +41,15 @@ from tests.inputs_util import GOLDEN
 pytestmark = pytest.mark.gpu
 
 
+@pytest.fixture(scope="module", autouse=True)
+def device_ready():
+    """One small context from the main thread before the worker-thread tests,
+    so the device runtime is initialised whatever order the files run in."""
+    s = synth.make_system(0)
+    with Context(s.pack(), precision="fp64", max_walkers=1) as c:
+        c.evaluate(np.arange(1), synth.initial_walkers(s, 1, 1))
+
+
 def _mcscf(name):
     with tempfile.TemporaryDirectory() as tmp:
         raw = os.path.join(tmp, name + ".ckmf")
@@ -49,7 +59,7 @@ def _mcscf(name):
     return packed
 
 
-@pytest.mark.parametrize("precision", ["fp32", "fp64", "fp32store", "bf16"])
+@pytest.mark.parametrize("precision", ["fp32", "fp64", "fp32store", "bf16", "bf16store"])
 @pytest.mark.parametrize("n_elec,W,scratch", [(16, 100, 0), (64, 300, 1 << 20)])
 def test_plan_equals_allocation(precision, n_elec, W, scratch):
     p = synth.make_system(-1, seed=11 + n_elec, n_elec=n_elec).pack()
@@ -151,7 +161,12 @@ def test_batched_dmc_config3_full_composition_past_step_100():
     with serving.BatchedDMC(comp, precision="fp32", workers=16, replace_cusps=True) as b:
         b.setup()
         b.advance(100)  # weights stay 1 for the first 100 steps
+        t0 = time.perf_counter()
         b.advance(400, rows, stats)
+        wall = time.perf_counter() - t0
+    moves = sum(st["n_moves"] for st in stats)
+    print("steps 100-499: %.3e walker-moves in %.2f s, %.3e walker-moves/s"
+          % (moves, wall, moves / wall), flush=True)
     tot = {k: sum(st[k] for st in stats) for k in
            ("n_split", "n_fused", "n_killed", "n_deleted", "n_split_skipped")}
     W = np.array([c[1] for c in comp])
