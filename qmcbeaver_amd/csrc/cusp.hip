@@ -70,7 +70,8 @@ cusp_kernel(CuspDev c, PointSet pts, GemmOut o) {
 #pragma unroll
     for (int k = 0; k < kComp; k++) {
       if (k == 0 && o.out_val) {
-        double *ov = o.out_val + wk * o.s_walker_val + pe + (long)m * o.s_orb;
+        double *ov = o.out_val + wk * o.s_walker_val + (p % o.per_walker) * o.s_elec_val +
+                     (long)m * o.s_orb_val;
         *ov += d[0];
         continue;
       }

@@ -144,7 +144,7 @@ mdet_move_kernel(MdetArgs A, const int *walkers, int num) {
       wc += v[0][o] * a;
 #pragma unroll
       for (int k = 0; k < 4; k++) {
-        const double x = (double)dm[k * cs + ro];
+        const double x = (double)dm[dm_at<TS>(k, ro, cs)];
         T1[k] += x * a;
         T2[k] += x * colr[o];
       }
@@ -446,7 +446,7 @@ mdet_commit_kernel(MdetArgs A, const int *walkers, const uint8_t *mask, int num)
     const int k = i / nact, o = i % nact;
     const double x = (double)vr[(long)k * A.v_s_comp + o];
     if (k == 0) dvw[(long)o * n + r] = x;
-    else dmw[(long)(k - 1) * cs + (long)o * n + r] = (TS)x;
+    else dmw[dm_at<TS>(k - 1, (long)o * n + r, cs)] = (TS)x;
   }
   double *Uxw = A.Ux + w0 * N * 3;
   const double *pUx = A.pUx + (long)b * N * 3;

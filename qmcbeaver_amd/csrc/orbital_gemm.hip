@@ -204,13 +204,13 @@ orbital_gemm_kernel(const TI *__restrict__ Ct, int ldc, int n_orb, const TI *__r
   const long wk = o.walker_map ? o.walker_map[row] : row;
   const long pe = (p % o.per_walker) * o.s_elec;
   if (o.out_val && comp == 0) {
-    double *ov = o.out_val + wk * o.s_walker_val + pe;
+    double *ov = o.out_val + wk * o.s_walker_val + (p % o.per_walker) * o.s_elec_val;
 #pragma unroll
     for (int t = 0; t < MT; t++)
 #pragma unroll
       for (int i = 0; i < 4; i++) {
         const int orb = o0 + 16 * t + M::row(lk, i);
-        if (orb < n_orb) ov[(long)orb * o.s_orb] = (double)acc[t][i];
+        if (orb < n_orb) ov[(long)orb * o.s_orb_val] = (double)acc[t][i];
       }
     return;
   }

@@ -426,8 +426,10 @@ struct Engine {
         TIMED("basis", (launch_basis<TC, TX>(ctx->dsys, pts, (TX *)Xs, ctx->kpad, ldp,
                                             ctx->stream)));
         const long na = ctx->nact[sp];
-        GemmOut go{ctx->dmat[sp], na * n, (long)n, (long)kDer * na * n, 1, n, d_map + c0,
-                   ctx->dval[sp], na * n};
+        // derivative components at dm_at<TS>(k, o * n + e, na * n); values FP64 (o, e)
+        const long e_s = dm_at<TS>(0, 1, 0), o_s = (long)n * e_s, k_s = dm_at<TS>(1, 0, na * n);
+        GemmOut go{ctx->dmat[sp], k_s, o_s, (long)kDer * na * n, e_s, n, d_map + c0,
+                   ctx->dval[sp], na * n, (long)n, 1};
         TIMED("orbital_gemm", (launch_orbital_gemm<TX, TS>((const TX *)ctx->Ct[sp], ctx->npad[sp],
                                                           (int)na,
                                                           (const TX *)Xs, ctx->kpad, ldp,
@@ -490,7 +492,7 @@ struct Engine {
                                         ctx->stream)));
     const int n = ctx->n[sp];
     GemmOut go{ctx->vrow, (long)ctx->npad[sp], 1, (long)kComp * ctx->npad[sp], 0, 1, nullptr,
-               nullptr, 0};
+               nullptr, 0, 0, 0};
     auto gemm = [&](int wpb) {
       return launch_orbital_gemm<TX, TO>((const TX *)ctx->Ct[sp], ctx->npad[sp], ctx->nact[sp],
                                          (const TX *)ctx->X, ctx->kpad, ctx->ldp, (long)num,

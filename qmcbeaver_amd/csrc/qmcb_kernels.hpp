@@ -5,6 +5,8 @@
 #include <hip/hip_bf16.h>
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "qmcb_common.hpp"
 
 namespace qmcb {
@@ -40,6 +42,18 @@ __device__ __forceinline__ double as_stored(double x) { return x; }
 template <>
 __device__ __forceinline__ double as_stored<bf16s>(double x) { return (double)bf16s(x); }
 
+// Offset of derivative component k (x, y, z, Laplacian) at ro = o * n + e in
+// one walker's stored derivative matrices (cs = rows * n per component).
+// FP64 / FP32: four component planes [4][rows][n].  BF16: the four
+// components of an element interleaved, [rows][n][4] (8 bytes), so that a
+// lane streaming a row reads them with one 8-byte load.
+template <typename TS>
+__host__ __device__ __forceinline__ long dm_at(int k, long ro, long cs) { return k * cs + ro; }
+template <>
+__host__ __device__ __forceinline__ long dm_at<bf16s>(int k, long ro, long cs) {
+  return (ro << 2) + k;
+}
+
 // A batch of evaluation points: point p lives at
 //   base + (p / per_walker) * walker_stride + (p % per_walker) * point_stride
 //   (with walker_map: base + walker_map[p / per_walker] * walker_stride + ...)
@@ -56,15 +70,16 @@ struct PointSet {
 // point p) is stored at
 //   out + k*s_comp + o*s_orb + (p / per_walker)*s_walker + (p % per_walker)*s_elec
 //   (the walker index goes through walker_map when it is non-null)
-//   With out_val set, component 0 goes to out_val (FP64, walker stride
-//   s_walker_val) and components 1..4 to out at (k-1)*s_comp.
+//   With out_val set, component 0 goes to out_val (FP64) at
+//   o*s_orb_val + walker*s_walker_val + (p % per_walker)*s_elec_val, and
+//   components 1..4 to out at (k-1)*s_comp.
 struct GemmOut {
   void *out;
   long s_comp, s_orb, s_walker, s_elec;
   int per_walker;
   const int *walker_map;
   double *out_val;
-  long s_walker_val;
+  long s_walker_val, s_orb_val, s_elec_val;
 };
 
 // cusp replacement added to GEMM output rows (device tables in qmcb_cusp.hpp)

@@ -601,10 +601,10 @@ traces_all_kernel(SlaterArgs A, const int *walkers, int num) {
     for (int o = 0; o < n; o++) {
       const double iv = inv[(long)o * n + e];
       const long ro = (long)occ_row(A, d, o) * n + e;
-      g0 += ld(dm + ro) * iv;
-      g1 += ld(dm + cs + ro) * iv;
-      g2 += ld(dm + 2 * cs + ro) * iv;
-      l += ld(dm + 3 * cs + ro) * iv;
+      g0 += ld(dm + dm_at<TS>(0, ro, cs)) * iv;
+      g1 += ld(dm + dm_at<TS>(1, ro, cs)) * iv;
+      g2 += ld(dm + dm_at<TS>(2, ro, cs)) * iv;
+      l += ld(dm + dm_at<TS>(3, ro, cs)) * iv;
     }
     if (!sing) {
       double *g = A.gradPR + ((long)w * n + e) * 3;
@@ -666,9 +666,9 @@ hipError_t launch_traces_all(const SlaterArgs &a, const int *walkers, int num, h
 #ifndef QMCB_SM_UNROLL_SMALL
 #define QMCB_SM_UNROLL_SMALL 2
 #endif
-// BF16 derivatives (16 bytes per row and lane): the FP32 depth measured best
-// (8.80 M walker-moves/s at 256 electrons against 8.70 / 8.42 / 8.21 M at 6 /
-// 8 / 10 rows)
+// BF16 derivatives (16 bytes per row and lane in two 8-byte loads): the FP32
+// depth measured best (9.41 M walker-moves/s at 256 electrons against 9.21 /
+// 8.97 M at 6 / 8 rows)
 #ifndef QMCB_SM_UNROLL_BF16
 #define QMCB_SM_UNROLL_BF16 4
 #endif
@@ -775,10 +775,27 @@ sm_move_kernel(MoveArgs A, const int *walkers, int num) {
   // ---- pass 1 set-up: the read-only stream's first U rows are requested
   // before column r is formed, so their latency overlaps it and the barrier
   const TS *dm = (const TS *)A.s.dmat + (long)w * kDer * nn2;
-  const TS *__restrict__ d1 = dm + c;
-  const TS *__restrict__ d2 = dm + 1 * nn2 + c;
-  const TS *__restrict__ d3 = dm + 2 * nn2 + c;
-  const TS *__restrict__ d4 = dm + 3 * nn2 + c;
+  const TS *__restrict__ d1 = dm + dm_at<TS>(0, c, nn2);
+  const TS *__restrict__ d2 = dm + dm_at<TS>(1, c, nn2);
+  const TS *__restrict__ d3 = dm + dm_at<TS>(2, c, nn2);
+  const TS *__restrict__ d4 = dm + dm_at<TS>(3, c, nn2);
+  // row o of a component is at d_k + DS * o * n (DS = 4 with BF16's
+  // interleaved components, read there with one 8-byte load)
+  constexpr long DS = std::is_same<TS, bf16s>::value ? 4 : 1;
+  auto ld4 = [&](long off, TS &x, TS &y, TS &z, TS &l) {
+    if constexpr (std::is_same<TS, bf16s>::value) {
+      const uint64_t q = ld_stream(reinterpret_cast<const uint64_t *>(d1 + DS * off));
+      x.b = (uint16_t)q;
+      y.b = (uint16_t)(q >> 16);
+      z.b = (uint16_t)(q >> 32);
+      l.b = (uint16_t)(q >> 48);
+    } else {
+      x = ld_stream(d1 + off);
+      y = ld_stream(d2 + off);
+      z = ld_stream(d3 + off);
+      l = ld_stream(d4 + off);
+    }
+  };
   // software pipeline in registers: the loads of the next U rows are issued
   // before the current U rows are consumed, so every wave keeps 2U rows
   // (U x 24/40 bytes per lane) in flight
@@ -792,10 +809,7 @@ sm_move_kernel(MoveArgs A, const int *walkers, int num) {
       const int o = min(o0 + u, o_end - 1);  // clamped: in bounds, unused past o_end
       const long off = (long)o * n;
       a[u] = ld_stream(ip_s + off);
-      x[u] = ld_stream(d1 + off);
-      y[u] = ld_stream(d2 + off);
-      z[u] = ld_stream(d3 + off);
-      l[u] = ld_stream(d4 + off);
+      ld4(off, x[u], y[u], z[u], l[u]);
     }
   };
   const bool stream = !flush && c < n;
@@ -840,9 +854,9 @@ sm_move_kernel(MoveArgs A, const int *walkers, int num) {
       double *ip = inv + c;
       for (int o = o_beg; o < o_end; o++) {
         const long off = (long)o * n;
-        ip[off] = row(o, ld_stream(ip + off), (double)ld_stream(d1 + off),
-                      (double)ld_stream(d2 + off), (double)ld_stream(d3 + off),
-                      (double)ld_stream(d4 + off));
+        TS x, y, z, l;
+        ld4(off, x, y, z, l);
+        ip[off] = row(o, ld_stream(ip + off), (double)x, (double)y, (double)z, (double)l);
       }
     }
   }
@@ -1077,7 +1091,7 @@ sm_move_kernel(MoveArgs A, const int *walkers, int num) {
     for (int i = t; i < kComp * n; i += NT) {
       const int k = i / n, o = i % n;
       if (k == 0) dvw[(long)o * n + r] = v[0][o];
-      else dmw[(long)(k - 1) * nn2 + (long)o * n + r] = (TS)v[k][o];
+      else dmw[dm_at<TS>(k - 1, (long)o * n + r, nn2)] = (TS)v[k][o];
     }
     double *Uxw = A.Ux + (long)w * N * 3;
     for (int i = t; i < 3 * N; i += NT) Uxw[i] = pUx[i];
@@ -1245,7 +1259,7 @@ commit_kernel(MoveArgs A, const int *walkers, const uint8_t *mask, int num) {
     const int k = i / n, o = i % n;
     const double x = (double)vr[(long)b * A.v_s_walker + (long)k * A.v_s_comp + o];
     if (k == 0) dvw[(long)o * n + r] = x;
-    else dmw[(long)(k - 1) * nn2 + (long)o * n + r] = (TS)x;
+    else dmw[dm_at<TS>(k - 1, (long)o * n + r, nn2)] = (TS)x;
   }
   double *Uxw = A.Ux + (long)w * N * 3;
   const double *pUx = A.pUx + (long)b * N * 3;
