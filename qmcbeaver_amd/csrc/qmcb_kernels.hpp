@@ -43,14 +43,13 @@ template <>
 __device__ __forceinline__ double as_stored<bf16s>(double x) { return (double)bf16s(x); }
 
 // Offset of derivative component k (x, y, z, Laplacian) at ro = o * n + e in
-// one walker's stored derivative matrices (cs = rows * n per component).
-// FP64 / FP32: four component planes [4][rows][n].  BF16: the four
-// components of an element interleaved, [rows][n][4] (8 bytes), so that a
-// lane streaming a row reads them with one 8-byte load.
+// one walker's stored derivative matrices ([rows][n][4]: the four components
+// of an element interleaved, so that a lane streaming a row reads them with
+// one 8-byte (BF16), one 16-byte (FP32) or two 16-byte (FP64) loads; cs, the
+// elements of one component, is kept for other layouts).
 template <typename TS>
-__host__ __device__ __forceinline__ long dm_at(int k, long ro, long cs) { return k * cs + ro; }
-template <>
-__host__ __device__ __forceinline__ long dm_at<bf16s>(int k, long ro, long cs) {
+__host__ __device__ __forceinline__ long dm_at(int k, long ro, long cs) {
+  (void)cs;
   return (ro << 2) + k;
 }
 

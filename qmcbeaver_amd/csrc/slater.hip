@@ -779,21 +779,24 @@ sm_move_kernel(MoveArgs A, const int *walkers, int num) {
   const TS *__restrict__ d2 = dm + dm_at<TS>(1, c, nn2);
   const TS *__restrict__ d3 = dm + dm_at<TS>(2, c, nn2);
   const TS *__restrict__ d4 = dm + dm_at<TS>(3, c, nn2);
-  // row o of a component is at d_k + DS * o * n (DS = 4 with BF16's
-  // interleaved components, read there with one 8-byte load)
-  constexpr long DS = std::is_same<TS, bf16s>::value ? 4 : 1;
+  // element (o, c) of the four components: 4 consecutive values at d1 + 4 o n
+  (void)d2; (void)d3; (void)d4;
   auto ld4 = [&](long off, TS &x, TS &y, TS &z, TS &l) {
     if constexpr (std::is_same<TS, bf16s>::value) {
-      const uint64_t q = ld_stream(reinterpret_cast<const uint64_t *>(d1 + DS * off));
+      const uint64_t q = ld_stream(reinterpret_cast<const uint64_t *>(d1 + 4 * off));
       x.b = (uint16_t)q;
       y.b = (uint16_t)(q >> 16);
       z.b = (uint16_t)(q >> 32);
       l.b = (uint16_t)(q >> 48);
+    } else if constexpr (std::is_same<TS, float>::value) {
+      typedef float f4 __attribute__((ext_vector_type(4)));
+      const f4 q = ld_stream(reinterpret_cast<const f4 *>(d1 + 4 * off));
+      x = q[0]; y = q[1]; z = q[2]; l = q[3];
     } else {
-      x = ld_stream(d1 + off);
-      y = ld_stream(d2 + off);
-      z = ld_stream(d3 + off);
-      l = ld_stream(d4 + off);
+      typedef double d2v __attribute__((ext_vector_type(2)));
+      const d2v q0 = ld_stream(reinterpret_cast<const d2v *>(d1 + 4 * off));
+      const d2v q1 = ld_stream(reinterpret_cast<const d2v *>(d1 + 4 * off + 2));
+      x = q0[0]; y = q0[1]; z = q1[0]; l = q1[1];
     }
   };
   // software pipeline in registers: the loads of the next U rows are issued
