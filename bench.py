@@ -45,7 +45,7 @@ def measured_traffic(precision):
         for line in open(PROFILE):
             if line.startswith("traffic_bytes_per_launch:"):
                 kv = dict(t.split("=") for t in line.split(":", 1)[1].split())
-                return float(kv["fp64" if precision == "fp64" else "mixed"])
+                return float(kv[{"fp64": "fp64", "bf16store": "bf16store"}.get(precision, "mixed")])
     except (OSError, KeyError, ValueError):
         pass
     return None
