@@ -3,16 +3,18 @@
 // Layout of the resident walker state in HBM (one context, capacity Wc):
 //   R        [Wc][N][3]           FP64 positions, alpha electrons first
 //   inv[s]   [Wc][n][n]           FP64 inverse Slater matrix, element (o,e)
-//   dmat[s]  [Wc][5][n][n]        Tstore, component k in {value, d/dx, d/dy,
-//                                 d/dz, Laplacian}, stored transposed:
-//                                 dmat(k, o, e) = d_k D(e, o)
+//   dval[s]  [Wc][n][n]           FP64 value matrix, dval(o, e) = D(e, o)
+//   dmat[s]  [Wc][n][n][4]        Tstore, the four components k in {d/dx,
+//                                 d/dy, d/dz, Laplacian} of an element
+//                                 together: dmat(o, e, k) = d_k D(e, o)
 //   logdet[s], sgn[s] [Wc]        FP64 ln|det|, sign (+1/-1)
 //   gradPR[s] [Wc][n][3]          FP64 grad det / det per electron
 //   lapPR[s]  [Wc]                FP64 Laplacian det / det
 //   U, Uxx, Vee, Ven [Wc]; Ux [Wc][N][3]   FP64 Jastrow / potential sums
 // The transposed derivative storage lets one thread own one electron column
-// of A^-1 and stream the matching column of every derivative matrix with
-// coalesced loads (64 lanes = 64 consecutive electrons).
+// of A^-1 and stream the matching column of the derivative matrices with
+// coalesced loads (64 lanes = 64 consecutive electrons), all four components
+// of an element in one load.
 #pragma once
 
 #include <hip/hip_runtime.h>

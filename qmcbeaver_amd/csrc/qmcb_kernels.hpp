@@ -111,7 +111,7 @@ struct SlaterArgs {
   int spin_start;        // first electron index of this spin
   double *inv;           // [Wc][n][n]
   double *dval;          // [Wc][n][n] FP64 value matrix, (o, e) = D(e, o)
-  void *dmat;            // [Wc][4][n][n] Tstore: d/dx, d/dy, d/dz, Laplacian
+  void *dmat;            // [Wc][n][n][4] Tstore: d/dx, d/dy, d/dz, Laplacian (dm_at)
   double *logdet, *sgn;  // [Wc]
   double *gradPR;        // [Wc][n][3]
   double *lapPR;         // [Wc]
@@ -123,7 +123,7 @@ struct SlaterArgs {
   // several determinants: the per-determinant arrays above (inv, logdet, sgn,
   // gradPR, lapPR, singular, dcount) are indexed by w * ndet + d, while dval /
   // dmat hold all nact active orbitals of walker w ([Wc][nact][n],
-  // [Wc][4][nact][n]); determinant d takes its column c from active orbital
+  // [Wc][nact][n][4]); determinant d takes its column c from active orbital
   // occ[d * n + c].  ndet <= 1, occ = null: one determinant, nact = n.
   int ndet, nact;
   const int *occ;

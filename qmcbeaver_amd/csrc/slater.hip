@@ -7,9 +7,10 @@
 // (Array2D.h:1566-1610).
 //
 // Storage (per walker, per spin): inv(o,e) row-major, dval(o,e) = D(e,o) in
-// FP64, dmat(k,o,e) = d_k D(e,o) for k in {x, y, z, Laplacian} (transposed),
-// so the thread that owns electron column e of A^-1 also streams column e of
-// every derivative matrix with coalesced loads.
+// FP64, dmat(o,e,k) = d_k D(e,o) for k in {x, y, z, Laplacian} (transposed,
+// the four components of an element together), so the thread that owns
+// electron column e of A^-1 also streams column e of the derivative matrices
+// with coalesced loads, one per element.
 #include "qmcb_kernels.hpp"
 
 namespace qmcb {
