@@ -157,6 +157,14 @@ typedef struct qmcb_options {
                                its saved record back (:259-262); one observable
                                row per step.  0: one electron per step, moved
                                in order (one_e_per_iter 1)                   */
+  int32_t graphs;           /* 1: with all_electron_moves, the evaluate at the
+                               proposal (basis, orbital GEMM, inverse, traces,
+                               Jastrow, assembly: no step-dependent argument)
+                               is captured in a HIP graph once two steps in a
+                               row share walker count, walker map and time
+                               step, and replayed while they do; 0: launched
+                               kernel by kernel.  Same results bit for bit;
+                               kernel timing launches kernel by kernel       */
 } qmcb_options;
 
 /* Caller-owned outputs, one row per evaluated walker.  Any pointer may be

@@ -46,7 +46,8 @@ class Options(C.Structure):
                 ("dt", C.c_double), ("umrigar_a", C.c_double),
                 ("seed", C.c_uint64), ("device", C.c_int32), ("replace_cusps", C.c_int32),
                 ("basis_scratch_bytes", C.c_int64), ("walker_groups", C.c_int32),
-                ("old_walker_acceptance", C.c_int32), ("all_electron_moves", C.c_int32)]
+                ("old_walker_acceptance", C.c_int32), ("all_electron_moves", C.c_int32),
+                ("graphs", C.c_int32)]
 
 
 class Results(C.Structure):

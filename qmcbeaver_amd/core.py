@@ -135,7 +135,7 @@ def cusp_fit(packed: Dict, spin: int = 0) -> np.ndarray:
 def _options(precision, max_walkers, drift_mod=QMCB_DRIFT_NONE, dt=0.1, umrigar_a=0.5,
              refresh_interval=0, seed=12345, device=0, replace_cusps=False,
              basis_scratch_bytes=0, walker_groups=0, old_walker_acceptance=50,
-             all_electron_moves=False) -> Options:
+             all_electron_moves=False, graphs=False) -> Options:
     o = Options()
     o.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
     o.max_walkers = max_walkers
@@ -150,6 +150,7 @@ def _options(precision, max_walkers, drift_mod=QMCB_DRIFT_NONE, dt=0.1, umrigar_
     o.walker_groups = int(walker_groups)
     o.old_walker_acceptance = int(old_walker_acceptance)
     o.all_electron_moves = 1 if all_electron_moves else 0
+    o.graphs = 1 if graphs else 0
     return o
 
 
@@ -189,7 +190,7 @@ class Context:
                  refresh_interval: int = 0, seed: int = 12345, device: int = 0,
                  replace_cusps: bool = False, basis_scratch_bytes: int = 0,
                  walker_groups: int = 0, old_walker_acceptance: int = 50,
-                 all_electron_moves: bool = False):
+                 all_electron_moves: bool = False, graphs: bool = False):
         """old_walker_acceptance: the reference's old_walker_acceptance_parameter
         (acceptance boost and DMC weight decay of walkers stuck for more than
         this many moves); 0 switches the rule off.  all_electron_moves: every
@@ -200,7 +201,7 @@ class Context:
         self.max_walkers = max_walkers
         o = _options(precision, max_walkers, drift_mod, dt, umrigar_a, refresh_interval, seed,
                      device, replace_cusps, basis_scratch_bytes, walker_groups,
-                     old_walker_acceptance, all_electron_moves)
+                     old_walker_acceptance, all_electron_moves, graphs)
         self._opt = o
         h = C.c_void_p()
         st = lib().qmcb_create(C.byref(self._buf.struct), C.byref(o), C.byref(h))

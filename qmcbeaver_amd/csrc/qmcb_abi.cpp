@@ -168,6 +168,13 @@ struct qmcb_ctx {
   void *scratch_dev = nullptr;   // this device's address of `scratch`
   // timing
   bool timing = false;
+  // all-electron moves: the evaluate at the proposal captured in a graph for
+  // this walker count, walker map and time step; ae_last_* is the previous
+  // step's, since a graph pays off only when the next step repeats it
+  hipGraphExec_t ae_graph = nullptr;
+  int ae_graph_num = -1, ae_last_num = -1;
+  const int *ae_graph_map = nullptr, *ae_last_map = nullptr;
+  double ae_graph_tau = 0.0, ae_last_tau = 0.0;
   Timing tm;
   std::string err;
 };
@@ -1192,6 +1199,7 @@ void qmcb_destroy(qmcb_ctx *ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (void *p : ctx->allocs) (void)hipFree(p);
   for (hipEvent_t e : ctx->tm.pool) (void)hipEventDestroy(e);
+  if (ctx->ae_graph) (void)hipGraphExecDestroy(ctx->ae_graph);
 
   for (int g = 0; g < qmcb_ctx::kMaxGroups; g++) {
     if (ctx->gstream[g]) {
@@ -1559,11 +1567,58 @@ static qmcb_status ae_begin(qmcb_ctx *ctx, const int *d_map, int num, const AeAr
   return QMCB_OK;
 }
 
+// The evaluate at the proposal: its launches (basis, orbital GEMM chunks,
+// cusp, inverse, traces, Jastrow, assembly) take no step-dependent argument,
+// so they are captured once per (num, walker map, tau) and replayed.  A step
+// whose key differs from the previous step's launches eagerly (a DMC
+// population that changes every step would otherwise capture every step);
+// so do kernel timing and graphs = 0 (the default: on this runtime a replay
+// measured no faster than the launches it holds).
+static qmcb_status ae_evaluate(qmcb_ctx *ctx, const int *d_map, int num, double tau) {
+  auto eager = [&]() -> qmcb_status {
+    qmcb_status s = DISPATCH(ctx, all_electron(ctx, d_map, num));
+    if (s != QMCB_OK) return s;
+    return DISPATCH(ctx, assemble(ctx, d_map, num, ctx->dout, tau, 0));
+  };
+  if (ctx->timing || !ctx->opt.graphs) return eager();
+  if (!(ctx->ae_graph && ctx->ae_graph_num == num && ctx->ae_graph_map == d_map &&
+        ctx->ae_graph_tau == tau)) {
+    const bool repeat = ctx->ae_last_num == num && ctx->ae_last_map == d_map &&
+                        ctx->ae_last_tau == tau;
+    ctx->ae_last_num = num;
+    ctx->ae_last_map = d_map;
+    ctx->ae_last_tau = tau;
+    if (!repeat) return eager();
+    if (ctx->ae_graph) {
+      (void)hipGraphExecDestroy(ctx->ae_graph);
+      ctx->ae_graph = nullptr;
+    }
+    QCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    const qmcb_status s = eager();
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (s != QMCB_OK || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      if (s != QMCB_OK) return s;
+      QCHK(e);
+    }
+    const hipError_t ie = hipGraphInstantiate(&ctx->ae_graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ie != hipSuccess) {
+      ctx->ae_graph = nullptr;
+      QCHK(ie);
+    }
+    ctx->ae_graph_num = num;
+    ctx->ae_graph_map = d_map;
+    ctx->ae_graph_tau = tau;
+  }
+  QCHK(hipGraphLaunch(ctx->ae_graph, ctx->stream));
+  return QMCB_OK;
+}
+
 static qmcb_status ae_step(qmcb_ctx *ctx, const int *d_map, int num, const AeArgs &a) {
   TIMED("ae_propose", launch_ae_propose(a, d_map, num, ctx->stream));
-  qmcb_status s = DISPATCH(ctx, all_electron(ctx, d_map, num));
-  if (s != QMCB_OK) return s;
-  s = DISPATCH(ctx, assemble(ctx, d_map, num, ctx->dout, a.tau, 0));
+  const qmcb_status s = ae_evaluate(ctx, d_map, num, a.tau);
   if (s != QMCB_OK) return s;
   TIMED("ae_accept", launch_ae_accept(a, d_map, num, ctx->stream));
   return QMCB_OK;

@@ -273,3 +273,38 @@ def test_allelectron_dmc_h2_reaches_exact_ground_state(modified):
     assert sm.decorr_depth >= 0 and sm.std_dev < 2e-3
     assert abs(sm.average - (-1.174476)) < max(4 * sm.std_dev, 1.5e-3)
     assert e_vmc > sm.average + 0.005
+
+
+@pytest.mark.parametrize("config,precision", [(0, "fp64"), (2, "fp32")])
+def test_allelectron_graph_replays_match_eager(config, precision):
+    """The evaluate at the proposal replayed from a captured HIP graph gives
+    the walk of the kernel-by-kernel launches bit for bit: positions and the
+    per-step observable rows after several calls, with an all-electron
+    evaluate and host synchronisations between them, then DMC steps whose
+    population changes (the graph re-captured or bypassed)."""
+    s = synth.make_system(config)
+    W = 128 if config == 0 else 256
+    R0 = synth.initial_walkers(s, W, 9)
+    print("case all-electron graph vs eager, config", config, precision, flush=True)
+    out = []
+    for graphs in (True, False):
+        with Context(s.pack(), precision=precision, max_walkers=2 * W, seed=77,
+                     all_electron_moves=True, graphs=graphs) as ctx:
+            ids = np.arange(W)
+            ctx.evaluate(ids, R0)
+            rec = []
+            for call in range(3):
+                st = ctx.vmc_steps(W, 5 * call, 5, 0.1 if config == 0 else 0.02)
+                rec.append((ctx.positions(W), ctx.sample_series(), st.sum_accept))
+                ctx.evaluate(ids, rec[-1][0])
+            ctx.dmc_init(W)
+            e0 = float(rec[-1][1][:, 0].sum() / rec[-1][1][:, 1].sum())
+            d = ctx.dmc_steps(15, 8, 0.01 if config == 0 else 0.002, e0, target=W,
+                              branch_threshold=1.2, fusion_threshold=0.8, reweight_after=0)
+            slots, w = ctx.dmc_population()
+            rec.append((ctx.positions(2 * W)[slots], ctx.sample_series(), d.population))
+            out.append(rec)
+    for a, b in zip(*out):
+        np.testing.assert_array_equal(a[0], b[0])
+        np.testing.assert_array_equal(a[1], b[1])
+        assert a[2] == b[2]
