@@ -488,9 +488,15 @@ def main():
             "workload": f"{s.name}: {N} e- ({n}+{s.n_dn}), {s.n_nuc} nuclei, {s.n_basis} "
                         f"Cartesian s/p/d basis functions, {args.walkers} walkers, "
                         f"single-electron importance-sampled VMC, tau={args.tau}",
-            "precision": "basis+orbital GEMM FP32 (MFMA), SM inverse/log-det/E_L FP64, "
-                         f"FP64 re-inversion every {args.refresh} moves"
-                         if args.precision != "fp64" else "FP64 throughout",
+            "precision": "FP64 throughout" if args.precision == "fp64" else
+                         ({"fp32": "basis FP64, orbital GEMM FP32 (MFMA), derivative matrices FP32",
+                           "fp32store": "basis and orbital GEMM FP64, derivative matrices FP32",
+                           "bf16": "basis FP64, orbital GEMM BF16 (MFMA, FP32 accumulate), "
+                                   "derivative matrices FP32",
+                           "bf16store": "basis FP64, orbital GEMM FP32 (MFMA), derivative "
+                                        "matrices BF16"}[args.precision]
+                          + f"; SM inverse/log-det/E_L FP64, FP64 re-inversion every "
+                            f"{args.refresh} moves"),
             "walkers": args.walkers, "electrons": N, "basis_functions": s.n_basis,
             "walker_groups": mixed["walker_groups"],
         },

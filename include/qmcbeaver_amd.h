@@ -43,7 +43,7 @@ typedef enum qmcb_precision {
                            derivative storage; FP64 re-inversion                  */
   QMCB_MIXED_BF16 = 2,  /* as MIXED_FP32 with BF16 GEMM operands (basis values
                            and coefficients rounded from FP64) on
-                           v_mfma_f32_16x16x16_bf16, FP32 accumulate            */
+                           v_mfma_f32_16x16x32_bf16, FP32 accumulate            */
   QMCB_MIXED_F32STORE = 3, /* FP64 basis and GEMM, derivative matrices stored
                            in FP32 (the HBM-bound part halves)                    */
   QMCB_MIXED_BF16STORE = 4 /* as MIXED_FP32 (FP32 MFMA GEMM), derivative matrices

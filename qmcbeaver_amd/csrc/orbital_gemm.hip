@@ -7,17 +7,19 @@
 //
 //   D^T[k][o][p] = sum_b  Ct[b][o] * X[k][b][p]
 //
-// Ct = C^T ([basis][orbital], zero padded to 16-multiples of orbitals and
-// 32-multiples of basis functions), X from kernel (a).  One wavefront owns a
-// 16-point tile and MT 16-orbital tiles; each k-step issues MT MFMA
-// instructions: 16x16x4 for FP64 (v_mfma_f64_16x16x4f64) and FP32
-// (v_mfma_f32_16x16x4f32) with lane l supplying A[l%16][l/16] and
-// B[l/16][l%16]; 16x16x16 for BF16 operands (v_mfma_f32_16x16x16_bf16,
-// FP32 accumulate) with lane l supplying the four k values 4(l/16)..+3 of
-// A[l%16][.] and B[.][l%16] (any k assignment shared by A and B gives the
-// same sum).  Accumulator element i (i = 0..3) of lane l is D[row][l%16]
-// with row = 4(l/16) + i for the FP32-output instructions and
-// row = 4i + l/16 for the FP64 one (Mfma<T>::row).
+// Ct = C^T ([basis][orbital], zero padded to the orbital tile widths and to
+// 32-multiples of basis functions; BF16 operands keep C orbital-major,
+// [orbital][basis], so a lane's eight k values of one orbital are one 16-byte
+// LDS read), X from kernel (a).  One wavefront owns a 16-point tile and MT
+// 16-orbital tiles; each k-step issues MT MFMA instructions: 16x16x4 for
+// FP64 (v_mfma_f64_16x16x4f64) and FP32 (v_mfma_f32_16x16x4f32) with lane l
+// supplying A[l%16][l/16] and B[l/16][l%16]; gfx950's 16x16x32 for BF16
+// operands (v_mfma_f32_16x16x32_bf16, FP32 accumulate) with lane l supplying
+// the eight k values 8(l/16)..+7 of A[l%16][.] and B[.][l%16] (any k
+// assignment shared by A and B gives the same sum).  Accumulator element i
+// (i = 0..3) of lane l is D[row][l%16] with row = 4(l/16) + i for the
+// FP32-output instructions and row = 4i + l/16 for the FP64 one
+// (Mfma<T>::row).
 #include <hip/hip_bf16.h>
 
 #include "qmcb_kernels.hpp"
@@ -26,7 +28,7 @@ namespace qmcb {
 
 typedef double d4v __attribute__((ext_vector_type(4)));
 typedef float f4v __attribute__((ext_vector_type(4)));
-typedef short s4v __attribute__((ext_vector_type(4)));
+typedef __bf16 b8v __attribute__((ext_vector_type(8)));
 
 template <typename TI>
 struct Mfma;
@@ -58,9 +60,9 @@ template <>
 struct Mfma<__hip_bfloat16> {
   typedef f4v acc_t;
   typedef float acc_s;
-  static constexpr int KPL = 4;
-  __device__ static acc_t run(s4v a, s4v b, acc_t c) {
-    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+  static constexpr int KPL = 8;
+  __device__ static acc_t run(b8v a, b8v b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
   }
   __device__ static constexpr int row(int lk, int i) { return 4 * lk + i; }
 };
@@ -70,7 +72,9 @@ struct Mfma<__hip_bfloat16> {
 // LDS, the next chunk is loaded from L2 into registers.  The point operand
 // X streams from HBM exactly once.  LDS rows are padded so that the four
 // k-rows one MFMA operand read touches (lanes 0-15 / 16-31 / ...) fall in
-// different banks.
+// different banks; BF16 chunks are stored orbital-major (KC + 8 elements a
+// row: the 16 orbitals of one 16-byte read start 20 banks apart and cover
+// the 64 banks once).
 template <typename TI>
 struct GemmTile {
   static constexpr int KC = 32;                       // k rows per chunk
@@ -85,12 +89,14 @@ orbital_gemm_kernel(const TI *__restrict__ Ct, int ldc, int n_orb, const TI *__r
   typedef typename M::acc_t acc_t;
   constexpr int KC = GemmTile<TI>::KC;
   constexpr int OW = 16 * MT;                    // orbital columns of the tile
-  constexpr int LW = OW + GemmTile<TI>::PAD;     // LDS row length
+  constexpr bool KT = M::KPL > 1;                // chunk stored orbital-major
+  constexpr int LR = KT ? OW : KC;               // LDS rows
+  constexpr int LW = KT ? KC + 8 : OW + GemmTile<TI>::PAD;  // LDS row length
   constexpr int VE = 16 / sizeof(TI);            // elements per 16-byte load
   constexpr int NT = 64 * WPB;
   constexpr int NV = (KC * OW + NT * VE - 1) / (NT * VE);  // 16-byte loads per thread, chunk
   static_assert(KC * OW % VE == 0, "tile / vector size mismatch");
-  __shared__ TI cs[2][KC][LW];
+  __shared__ TI cs[2][LR][LW];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const long p0 = ((long)blockIdx.x * WPB + wave) * 16;
@@ -102,20 +108,22 @@ orbital_gemm_kernel(const TI *__restrict__ Ct, int ldc, int n_orb, const TI *__r
 #pragma unroll
   for (int t = 0; t < MT; t++) acc[t] = acc_t{0, 0, 0, 0};
 
-  // cooperative chunk load: element e = (j * NT + tid) * VE of the KC x OW tile
+  // cooperative chunk load: element e = (j * NT + tid) * VE of the KC x OW
+  // tile (OW x KC for orbital-major BF16, whose rows are kpad long)
   typedef __attribute__((ext_vector_type(4))) unsigned int u4;
   u4 stage[NV];
   auto gload = [&](int k0) {
 #pragma unroll
     for (int j = 0; j < NV; j++) {
-      const int e = (j * NT + tid) * VE, r = e / OW, c = e % OW;
-      if (e < KC * OW) stage[j] = *(const u4 *)(Ct + (long)(k0 + r) * ldc + o0 + c);
+      const int e = (j * NT + tid) * VE, r = e / (LR == KC ? OW : KC), c = e % (LR == KC ? OW : KC);
+      const long src = KT ? (long)(o0 + r) * kpad + k0 + c : (long)(k0 + r) * ldc + o0 + c;
+      if (e < KC * OW) stage[j] = *(const u4 *)(Ct + src);
     }
   };
   auto lstore = [&](int buf) {
 #pragma unroll
     for (int j = 0; j < NV; j++) {
-      const int e = (j * NT + tid) * VE, r = e / OW, c = e % OW;
+      const int e = (j * NT + tid) * VE, r = e / (LR == KC ? OW : KC), c = e % (LR == KC ? OW : KC);
       if (e < KC * OW) *(u4 *)&cs[buf][r][c] = stage[j];
     }
   };
@@ -155,15 +163,12 @@ orbital_gemm_kernel(const TI *__restrict__ Ct, int ldc, int n_orb, const TI *__r
       } else {
 #pragma unroll
         for (int s = 0; s < KC / KS; s++) {
-          s4v bv;
+          b8v bv;
 #pragma unroll
-          for (int j = 0; j < KPL; j++) bv[j] = __builtin_bit_cast(short, b[s * KPL + j]);
+          for (int j = 0; j < KPL; j++) bv[j] = __builtin_bit_cast(__bf16, b[s * KPL + j]);
 #pragma unroll
           for (int t = 0; t < MT; t++) {
-            s4v av;
-#pragma unroll
-            for (int j = 0; j < KPL; j++)
-              av[j] = __builtin_bit_cast(short, cs[buf][KS * s + KPL * lk + j][16 * t + li]);
+            const b8v av = *(const b8v *)&cs[buf][16 * t + li][KS * s + KPL * lk];
             acc[t] = M::run(av, bv, acc[t]);
           }
         }

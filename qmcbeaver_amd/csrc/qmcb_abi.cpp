@@ -117,7 +117,8 @@ struct qmcb_ctx {
   // basis scratch
   void *X = nullptr;
   long ldp = 0;      // point capacity (multiple of 64)
-  // coefficients, transposed [kpad][npad] per spin, in the GEMM operand type
+  // coefficients per spin in the GEMM operand type: transposed [kpad][npad]
+  // (FP64, FP32), orbital-major [npad][kpad] (BF16)
   void *Ct[2] = {nullptr, nullptr};
   // device output arrays (batch rows) for the host-buffer entry points
   qmcb_results dout{};
@@ -834,8 +835,12 @@ qmcb_status create_impl(const qmcb_system *s, const qmcb_options *o, qmcb_ctx *c
       QCHK(hipMemcpyAsync(ctx->Ct[sp], ct.data(), cnt * 8, hipMemcpyHostToDevice, ctx->stream));
       QCHK(hipStreamSynchronize(ctx->stream));
     } else if (ctx->tx == 2) {
+      // BF16 operands orbital-major, [npad][kpad] (the GEMM's 16x16x32 MFMA
+      // takes eight consecutive basis functions of one orbital per lane)
       std::vector<uint16_t> cb(cnt);
-      for (size_t i = 0; i < cnt; i++) cb[i] = bf16_bits((float)ct[i]);
+      const long np = ctx->npad[sp], kp = ctx->kpad;
+      for (long o2 = 0; o2 < np; o2++)
+        for (long b = 0; b < kp; b++) cb[o2 * kp + b] = bf16_bits((float)ct[b * np + o2]);
       QCHK(hipMemcpyAsync(ctx->Ct[sp], cb.data(), cnt * 2, hipMemcpyHostToDevice, ctx->stream));
       QCHK(hipStreamSynchronize(ctx->stream));
     } else {
