@@ -19,10 +19,6 @@
 
 namespace qmcb {
 
-__device__ __forceinline__ int ee_type(int i, int j, int n_up) {
-  const bool ui = i < n_up, uj = j < n_up;
-  return (ui != uj) ? 0 : (ui ? 1 : 2);
-}
 
 template <int NT>
 __device__ __forceinline__ void block_sum(double *vals, int nv, double *scratch /*[nv][NT/64]*/) {

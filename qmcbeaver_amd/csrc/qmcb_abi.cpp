@@ -132,6 +132,7 @@ struct qmcb_ctx {
   bool multi = false;
   int nact[2] = {0, 0};
   double *d_ci = nullptr;
+  const qmcb_pade *d_ee = nullptr;  // DevSystem::ee on the device
   int *d_occ[2] = {nullptr, nullptr};
   int *st_rowsing = nullptr;  // [Wc] combined proposal failed
   double *cgrad = nullptr;     // [Wc][N][3] grad ln D of the determinant sum
@@ -406,6 +407,14 @@ qmcb_status tune_gemm(qmcb_ctx *ctx, int sp, int num, F &&gemm) {
   return QMCB_OK;
 }
 
+// Whether the device walk's single-electron move of electron m lets sm_move
+// form the Jastrow factor at the proposal (one determinant, N no more than the
+// move tile's threads) instead of a jastrow_move launch before it.
+static bool jastrow_fused(const qmcb_ctx *ctx, int m) {
+  const int sp = m < ctx->n[0] ? 0 : 1;
+  return !ctx->multi && sm_move_fuses_jastrow(ctx->n[sp], ctx->N);
+}
+
 // ---- precision dispatch ------------------------------------------------------
 // TS: derivative storage, TX: basis/GEMM operand, TO: GEMM output
 template <typename TS, typename TX, typename TC, typename TO>
@@ -484,8 +493,10 @@ struct Engine {
     return QMCB_OK;
   }
 
-  // proposal rows: propR[b] set; Jastrow at the proposal, basis + GEMM row
-  static qmcb_status move_front(qmcb_ctx *ctx, const int *d_map, int num, int m) {
+  // proposal rows: propR[b] set; Jastrow at the proposal (unless sm_move
+  // forms it, jastrow = false), basis + GEMM row
+  static qmcb_status move_front(qmcb_ctx *ctx, const int *d_map, int num, int m,
+                                bool jastrow = true) {
     const int sp = m < ctx->n[0] ? 0 : 1;
     JastrowArgs ja{};
     ja.sys = ctx->dsys;
@@ -493,8 +504,9 @@ struct Engine {
     ja.U = ctx->pU; ja.Ux = ctx->pUx; ja.Uxx = ctx->pUxx; ja.Vee = ctx->pVee; ja.Ven = ctx->pVen;
     ja.moved = m;
     ja.propR = ctx->propR;
-    TIMED("jastrow_move", launch_jastrow_move(ja, ctx->U, ctx->Ux, ctx->Uxx, ctx->Vee, ctx->Ven,
-                                              d_map, num, ctx->stream));
+    if (jastrow)
+      TIMED("jastrow_move", launch_jastrow_move(ja, ctx->U, ctx->Ux, ctx->Uxx, ctx->Vee,
+                                                ctx->Ven, d_map, num, ctx->stream));
     PointSet pts{ctx->propR, (long)num, 1, 3, 0, nullptr};
     TIMED("basis", (launch_basis<TC, TX>(ctx->dsys, pts, (TX *)ctx->X, ctx->kpad, ctx->ldp,
                                         ctx->stream)));
@@ -536,6 +548,9 @@ struct Engine {
     a.U = ctx->U; a.Ux = ctx->Ux; a.Uxx = ctx->Uxx; a.Vee = ctx->Vee; a.Ven = ctx->Ven;
     a.pU = ctx->pU; a.pUx = ctx->pUx; a.pUxx = ctx->pUxx; a.pVee = ctx->pVee; a.pVen = ctx->pVen;
     a.v_nn = ctx->dsys.v_nn;
+    a.ee = ctx->d_ee;
+    a.en = ctx->dsys.en[sp];
+    a.n_up = ctx->n[0];
     a.drift_mod = ctx->opt.drift_mod;
     a.dt = ctx->opt.dt;
     a.umrigar_a = ctx->opt.umrigar_a;
@@ -775,6 +790,7 @@ qmcb_status create_impl(const qmcb_system *s, const qmcb_options *o, qmcb_ctx *c
   d.ee[2] = s->ee_dndn;
   QCHK(upload(ctx, &d.en[0], s->en_up, (size_t)ctx->nn));
   QCHK(upload(ctx, &d.en[1], s->en_dn, (size_t)ctx->nn));
+  QCHK(upload(ctx, &ctx->d_ee, d.ee, 3));  // for kernels that take no DevSystem
   double vnn = 0.0;
   for (int i = 0; i < ctx->nn; i++)
     for (int j = i + 1; j < ctx->nn; j++) {
@@ -1058,6 +1074,7 @@ size_t plan_device_bytes(const qmcb_system *s, const qmcb_options *o) {
   add(s->n_prim, 8);
   add(nn, sizeof(qmcb_pade));
   add(nn, sizeof(qmcb_pade));
+  add(3, sizeof(qmcb_pade));
   const bool multi = s->n_det > 1 || s->occ_up || s->occ_dn;
   const long ndet = std::max(1, (int)s->n_det);
   long nact[2];
@@ -1711,11 +1728,13 @@ qmcb_status qmcb_vmc_steps(qmcb_ctx *ctx, int32_t num, int64_t first_step, int32
           // Sherman-Morrison pass; group 0 runs free
           if (g > 0) QCHK(hipStreamWaitEvent(ctx->stream, ctx->gfront[g - 1], 0));
           st = propose(step, m, d_map, (int)cnt);
-          if (st == QMCB_OK) st = DISPATCH(ctx, move_front(ctx, d_map, (int)cnt, m));
+          if (st == QMCB_OK)
+            st = DISPATCH(ctx, move_front(ctx, d_map, (int)cnt, m, !jastrow_fused(ctx, m)));
           if (st == QMCB_OK) QCHK(hipEventRecord(ctx->gfront[g], ctx->stream));
           if (st == QMCB_OK) {
             MoveArgs a = DISPATCH(ctx, move_args(ctx, m, 1, none));
             a.tau = tau;
+            a.jastrow_fused = jastrow_fused(ctx, m);
             if (m == 0) a.obs = ctx->d_obs;
             st = DISPATCH(ctx, sm(ctx, a, d_map, (int)cnt));
           }
@@ -1737,7 +1756,7 @@ qmcb_status qmcb_vmc_steps(qmcb_ctx *ctx, int32_t num, int64_t first_step, int32
       }
       st = propose(step, m, nullptr, num);
       if (st != QMCB_OK) return st;
-      st = DISPATCH(ctx, move_front(ctx, nullptr, num, m));
+      st = DISPATCH(ctx, move_front(ctx, nullptr, num, m, !jastrow_fused(ctx, m)));
       if (st != QMCB_OK) return st;
       if (ctx->multi) {
         // several determinants: stage every determinant, combine and accept on
@@ -1749,6 +1768,7 @@ qmcb_status qmcb_vmc_steps(qmcb_ctx *ctx, int32_t num, int64_t first_step, int32
       } else {
         MoveArgs a = DISPATCH(ctx, move_args(ctx, m, 1, none));
         a.tau = tau;
+        a.jastrow_fused = jastrow_fused(ctx, m);
         if (m == 0) a.obs = ctx->d_obs;
         st = DISPATCH(ctx, sm(ctx, a, nullptr, num));
       }
@@ -2074,7 +2094,7 @@ qmcb_status qmcb_dmc_steps(qmcb_ctx *ctx, int64_t first_step, int32_t n_steps,
       pa.gauss = ctx->gauss;
       pa.unif = ctx->unif;
       TIMED("propose", launch_propose(pa, d_map, cnt, ctx->stream));
-      qmcb_status gst = DISPATCH(ctx, move_front(ctx, d_map, cnt, m));
+      qmcb_status gst = DISPATCH(ctx, move_front(ctx, d_map, cnt, m, !jastrow_fused(ctx, m)));
       if (gst != QMCB_OK) return gst;
       if (g >= 0) QCHK(hipEventRecord(ctx->gfront[g], ctx->stream));
       qmcb_results none{};
@@ -2097,6 +2117,7 @@ qmcb_status qmcb_dmc_steps(qmcb_ctx *ctx, int64_t first_step, int32_t n_steps,
       } else {
         MoveArgs a = DISPATCH(ctx, move_args(ctx, m, 2, none));
         a.tau = prm->tau;
+        a.jastrow_fused = jastrow_fused(ctx, m);
         a.e_trial = ctx->e_trial;
         a.e_est = ctx->e_est;
         a.energy_mod = prm->energy_modification;

@@ -74,6 +74,12 @@ __device__ __forceinline__ void pade_eval(const qmcb_pade &f, double r, double &
   d2u = (app - 2.0 * du * bp - u * bpp) * ib;
 }
 
+// e-e Pade type of the pair (i, j): 0 up-dn, 1 up-up, 2 dn-dn (DevSystem::ee)
+__device__ __forceinline__ int ee_type(int i, int j, int n_up) {
+  const bool ui = i < n_up, uj = j < n_up;
+  return (ui != uj) ? 0 : (ui ? 1 : 2);
+}
+
 // ---- walker bookkeeping of the reference's QMCWalker ------------------------
 // Acceptance of an old walker (QMCWalker.cpp:1578-1600): after more than
 // old_walker consecutive rejected moves the probability is raised by

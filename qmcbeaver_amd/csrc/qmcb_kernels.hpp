@@ -157,6 +157,13 @@ struct MoveArgs {
   double *U, *Ux, *Uxx, *Vee, *Ven;           // current Jastrow/potential
   const double *pU, *pUx, *pUxx, *pVee, *pVen;  // at the proposal
   double v_nn;
+  // jastrow_fused: sm_move forms the Jastrow factor and potential at the
+  // proposal itself (the moved electron's pairs; needs N <= the tile's
+  // threads, sm_move_fuses_jastrow) instead of reading pU ... pVen
+  int jastrow_fused;
+  const qmcb_pade *ee;   // [3] up-dn, up-up, dn-dn
+  const qmcb_pade *en;   // [n_nuc] of the moved spin
+  int n_up;
   int drift_mod;
   double dt, umrigar_a;
   // outputs at the proposal (device, any may be null)
@@ -245,6 +252,10 @@ hipError_t launch_copy_walkers(const CopyArgs &a, const int *pairs /*[2*np]*/, i
 
 template <typename TS, typename TO>
 hipError_t launch_sm_move(const MoveArgs &a, const int *walkers, int num, hipStream_t st);
+// whether sm_move can form the Jastrow factor at the proposal itself for N
+// electrons when n electrons of the moved spin set its tile (one electron
+// per thread)
+bool sm_move_fuses_jastrow(int n, int N);
 
 // ---- kernel (d): Jastrow + Coulomb, all-electron and single-electron ----
 struct JastrowArgs {

@@ -749,6 +749,8 @@ sm_move_kernel(MoveArgs A, const int *walkers, int num) {
   __shared__ double gq[4][NC];
   __shared__ double wred5[8][NW];
   __shared__ double s_bc[8];
+  __shared__ double s_ux[NT][3];  // jastrow_fused: electron t's Jastrow gradient at the proposal
+  __shared__ double wj[7][NW];    // jastrow_fused: per-wave sums
 
   const TO *vr = (const TO *)A.vrow + (long)b * A.v_s_walker;
   double *inv = A.s.inv + (long)w * nn2;
@@ -771,6 +773,61 @@ sm_move_kernel(MoveArgs A, const int *walkers, int num) {
   // first batch instead of after its barrier
   static_assert(NT >= NC, "one column element per thread");
   const double xr = t < n ? inv[(long)t * n + r] : 0.0;
+  if (A.jastrow_fused) {
+    // the Jastrow factor and potential at the proposal, the terms of
+    // jastrow_move_kernel (N <= NT): thread t takes electron t's pair with
+    // the moved electron at its old and new position, and nucleus t; the
+    // sums dU, dUxx, dVee, dVen, grad_m U go to wj per wave, electron t's
+    // new gradient to s_ux (before the stream, whose in-flight rows would
+    // otherwise hold registers across it)
+    const int m = A.moved;
+    const double *Rw = A.R + (long)w * N * 3;
+    const double xo[3] = {Rw[3 * m], Rw[3 * m + 1], Rw[3 * m + 2]};
+    const double xn[3] = {A.propR[3L * b], A.propR[3L * b + 1], A.propR[3L * b + 2]};
+    double js[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (t < N && t != m) {
+      double dn[3], dold[3], rn2 = 0.0, ro2 = 0.0;
+      for (int k = 0; k < 3; k++) {
+        dn[k] = xn[k] - Rw[3 * t + k]; rn2 += dn[k] * dn[k];
+        dold[k] = xo[k] - Rw[3 * t + k]; ro2 += dold[k] * dold[k];
+      }
+      const double rn = sqrt(rn2), ro = sqrt(ro2), irn = 1.0 / rn, iro = 1.0 / ro;
+      const qmcb_pade &f = A.ee[ee_type(m, t, A.n_up)];
+      double un, dun, d2un, uo, duo, d2uo;
+      pade_eval(f, rn, un, dun, d2un);
+      pade_eval(f, ro, uo, duo, d2uo);
+      js[0] = un - uo;
+      js[1] = 2.0 * (2.0 * irn * dun + d2un) - 2.0 * (2.0 * iro * duo + d2uo);
+      js[2] = irn - iro;
+      const double gn = dun * irn, go = duo * iro;
+      const double *uxw = A.Ux + (long)w * N * 3;
+      for (int k = 0; k < 3; k++) {
+        s_ux[t][k] = uxw[3 * t + k] - gn * dn[k] + go * dold[k];
+        js[4 + k] = gn * dn[k];
+      }
+    }
+    for (int a_ = t; a_ < A.n_nuc; a_ += NT) {
+      double dn[3], dold[3], rn2 = 0.0, ro2 = 0.0;
+      for (int k = 0; k < 3; k++) {
+        dn[k] = xn[k] - A.nuc_xyz[3 * a_ + k]; rn2 += dn[k] * dn[k];
+        dold[k] = xo[k] - A.nuc_xyz[3 * a_ + k]; ro2 += dold[k] * dold[k];
+      }
+      const double rn = sqrt(rn2), ro = sqrt(ro2), irn = 1.0 / rn, iro = 1.0 / ro;
+      double un, dun, d2un, uo, duo, d2uo;
+      pade_eval(A.en[a_], rn, un, dun, d2un);
+      pade_eval(A.en[a_], ro, uo, duo, d2uo);
+      js[0] += un - uo;
+      js[1] += (2.0 * irn * dun + d2un) - (2.0 * iro * duo + d2uo);
+      js[3] += A.nuc_z[a_] * (iro - irn);
+      const double gn = dun * irn;
+      for (int k = 0; k < 3; k++) js[4 + k] += gn * dn[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 7; k++) {
+      const double s = wave_sum(js[k]);
+      if ((t & 63) == 0) wj[k][t >> 6] = s;
+    }
+  }
   __syncthreads();
 
   // ---- pass 1 set-up: the read-only stream's first U rows are requested
@@ -905,6 +962,13 @@ sm_move_kernel(MoveArgs A, const int *walkers, int num) {
   // ---- assembly over all electrons
   const int so = A.other.spin_start, no = A.other.n, ss = A.s.spin_start;
   const double *pUx = A.pUx + (long)b * N * 3;
+  // jastrow_fused: sum k of the Jastrow terms at the proposal (dU, dUxx,
+  // dVee, dVen, grad_m U), read where used so none stays live in registers
+  auto jsum = [&](int k) {
+    double s = 0.0;
+    for (int q = 0; q < NW; q++) s += wj[k][q];
+    return s;
+  };
   const double *cUx = A.Ux + (long)w * N * 3;
   const double *Rw = A.R + (long)w * N * 3;
   const double *gpo = no > 0 ? A.other.gradPR + (long)w * no * 3 : nullptr;
@@ -923,7 +987,8 @@ sm_move_kernel(MoveArgs A, const int *walkers, int num) {
     }
     double g[3], gc[3];
     for (int k = 0; k < 3; k++) {
-      const double ux = pUx[3 * e + k];
+      const double ux = !A.jastrow_fused ? pUx[3 * e + k]
+                                         : (e == A.moved ? jsum(4 + k) : s_ux[e][k]);
       g[k] = dx[k] + ux;
       gc[k] = dxc[k] + cUx[3 * e + k];
       lsum += ux * dx[k] * 2.0;
@@ -993,11 +1058,15 @@ sm_move_kernel(MoveArgs A, const int *walkers, int num) {
   const double mr_o = (qf && s_bc[7] > 0.0) ? sqrt(s_bc[6]) / sqrt(s_bc[7]) : 1.0;
   const double lap_s = s_bc[3];
   const double lap_o = no > 0 ? A.other.lapPR[w] : 0.0;
-  const double lapR = lap_s + lap_o + A.pUxx[b] + s_bc[0];
-  const double V = A.v_nn + A.pVee[b] + A.pVen[b];
+  const double pU_b = A.jastrow_fused ? A.U[w] + jsum(0) : A.pU[b];
+  const double pUxx_b = A.jastrow_fused ? A.Uxx[w] + jsum(1) : A.pUxx[b];
+  const double pVee_b = A.jastrow_fused ? A.Vee[w] + jsum(2) : A.pVee[b];
+  const double pVen_b = A.jastrow_fused ? A.Ven[w] + jsum(3) : A.pVen[b];
+  const double lapR = lap_s + lap_o + pUxx_b + s_bc[0];
+  const double V = A.v_nn + pVee_b + pVen_b;
   const double EL = -0.5 * lapR + V;
   const double ld_s = A.s.logdet[w], ld_o = no > 0 ? A.other.logdet[w] : 0.0;
-  const double logpsi_new = ld_s + log(fabs(ratio)) + ld_o + A.pU[b];
+  const double logpsi_new = ld_s + log(fabs(ratio)) + ld_o + pU_b;
   const double sg_o = no > 0 ? A.other.sgn[w] : 1.0;
   const double sgn_s_new = A.s.sgn[w] * (ratio < 0 ? -1.0 : 1.0);
   bool sing = !ok || A.s.singular[w] || (no > 0 && A.other.singular[w]);
@@ -1031,10 +1100,10 @@ sm_move_kernel(MoveArgs A, const int *walkers, int num) {
     if (o.e_local) o.e_local[b] = EL;
     if (o.kinetic) o.kinetic[b] = -0.5 * lapR;
     if (o.potential) o.potential[b] = V;
-    if (o.v_ne) o.v_ne[b] = A.pVen[b];
-    if (o.v_ee) o.v_ee[b] = A.pVee[b];
+    if (o.v_ne) o.v_ne[b] = pVen_b;
+    if (o.v_ee) o.v_ee[b] = pVee_b;
     if (o.det_ratio) o.det_ratio[b] = ratio;
-    if (o.jastrow_u) o.jastrow_u[b] = A.pU[b];
+    if (o.jastrow_u) o.jastrow_u[b] = pU_b;
     if (o.singular) o.singular[b] = sing ? 1 : 0;
   }
 
@@ -1098,7 +1167,12 @@ sm_move_kernel(MoveArgs A, const int *walkers, int num) {
       else dmw[dm_at<TS>(k - 1, (long)o * n + r, nn2)] = (TS)v[k][o];
     }
     double *Uxw = A.Ux + (long)w * N * 3;
-    for (int i = t; i < 3 * N; i += NT) Uxw[i] = pUx[i];
+    if (A.jastrow_fused) {
+      if (t < N)
+        for (int k = 0; k < 3; k++) Uxw[3 * t + k] = t == A.moved ? jsum(4 + k) : s_ux[t][k];
+    } else {
+      for (int i = t; i < 3 * N; i += NT) Uxw[i] = pUx[i];
+    }
     if (t == 0) {
       A.s.dcount[w] = base + 1;
       A.s.lapPR[w] = lap_s;
@@ -1106,10 +1180,10 @@ sm_move_kernel(MoveArgs A, const int *walkers, int num) {
       A.s.sgn[w] = sgn_s_new;
       double *Rm = A.R + ((long)w * N + A.moved) * 3;
       Rm[0] = A.propR[3L * b]; Rm[1] = A.propR[3L * b + 1]; Rm[2] = A.propR[3L * b + 2];
-      A.U[w] = A.pU[b];
-      A.Uxx[w] = A.pUxx[b];
-      A.Vee[w] = A.pVee[b];
-      A.Ven[w] = A.pVen[b];
+      A.U[w] = pU_b;
+      A.Uxx[w] = pUxx_b;
+      A.Vee[w] = pVee_b;
+      A.Ven[w] = pVen_b;
       A.cur_el[w] = EL;
     }
   } else if (flush && t == 0) {
@@ -1213,11 +1287,25 @@ hipError_t launch_copy_walkers(const CopyArgs &a, const int *pairs, int np, hipS
   return hipGetLastError();
 }
 
+// threads of the tile that serves n electrons of the moved spin
+static int sm_move_threads(int n) {
+  if (n <= 16) return 64;
+  if (n <= 32) return QMCB_SM_NT32;
+  if (n <= 64) return QMCB_SM_NT64;
+  if (n <= 96 && QMCB_SM_TILE96) return 192;
+  return 256;
+}
+
+bool sm_move_fuses_jastrow(int n, int N) {
+  return n > 0 && n <= 128 && N <= sm_move_threads(n);
+}
+
 template <typename TS, typename TO>
 hipError_t launch_sm_move(const MoveArgs &a, const int *walkers, int num, hipStream_t st) {
   if (num <= 0) return hipSuccess;
   const int n = a.s.n;
   if (n <= 0) return hipErrorInvalidValue;
+  if (a.jastrow_fused && !sm_move_fuses_jastrow(n, a.N)) return hipErrorInvalidValue;
   if (n <= 16)
     hipLaunchKernelGGL((sm_move_kernel<TS, TO, 16, 64>), dim3(num), dim3(64), 0, st, a, walkers,
                        num);
