@@ -193,7 +193,11 @@ int32_t qmcb_abi_version(void);
 
 /* Build a context: copies `sys` to the device and allocates the resident
  * walker state.  Replaces QMCFunctionsFactory::functionsFactory +
- * QMCSCFJastrow::initialize (QMCSCFJastrow.cpp:61-89). */
+ * QMCSCFJastrow::initialize (QMCSCFJastrow.cpp:61-89).  The environment
+ * variable QMCB_GEMM_TILE="wpb,mt" (wpb 2 or 4 waves per workgroup, mt 1, 2,
+ * 4 or 8 orbital tiles of 16 per wave), read here, fixes the single-electron
+ * orbital GEMM's tile instead of the timing autotuner; every tile gives the
+ * same bits. */
 qmcb_status qmcb_create(const qmcb_system *sys, const qmcb_options *opt,
                         qmcb_ctx **out);
 

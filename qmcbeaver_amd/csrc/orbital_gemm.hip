@@ -242,57 +242,76 @@ orbital_gemm_kernel(const TI *__restrict__ Ct, int ldc, int n_orb, const TI *__r
 // launch) stream X from HBM, where reading it once with 128-column tiles is
 // faster (1418 against 1702 us FP32 per 88320-row chunk).
 constexpr long kSplitRows = 16384;
+int gemm_orbital_tiles(int n_orb, long P) {
+  if (n_orb > 64 && P > kSplitRows) return 8;
+  if (n_orb > 32) return 4;
+  if (n_orb > 16) return 2;
+  return 1;
+}
+
+bool gemm_orbital_tiles_valid(int mt, int n_orb, int ldc) {
+  if (mt != 1 && mt != 2 && mt != 4 && mt != 8) return false;
+  const int ow = 16 * mt;
+  return (long)((n_orb + ow - 1) / ow) * ow <= ldc;  // every tile inside the padded orbitals
+}
+
 template <typename TI, typename TO, int WPB>
 static void gemm_grid(const TI *Ct, int ldc, int n_orb, const TI *X, int ldk, long ldp, long P,
-                      int kpad, const GemmOut &o, hipStream_t st) {
+                      int kpad, const GemmOut &o, hipStream_t st, int mt) {
   const unsigned gx = (unsigned)((P + 16 * WPB - 1) / (16 * WPB));
   const dim3 blk(64 * WPB);
-  if (n_orb > 64 && P > kSplitRows) {
+  if (mt == 8) {
     hipLaunchKernelGGL((orbital_gemm_kernel<TI, TO, 8, WPB>),
                        dim3(gx, kComp, (unsigned)((n_orb + 127) / 128)), blk, 0, st, Ct, ldc,
                        n_orb, X, ldk, ldp, P, kpad, o);
-  } else if (n_orb > 32) {
+  } else if (mt == 4) {
     hipLaunchKernelGGL((orbital_gemm_kernel<TI, TO, 4, WPB>),
                        dim3(gx, kComp, (unsigned)((n_orb + 63) / 64)), blk, 0, st, Ct, ldc, n_orb,
                        X, ldk, ldp, P, kpad, o);
-  } else if (n_orb > 16) {
-    hipLaunchKernelGGL((orbital_gemm_kernel<TI, TO, 2, WPB>), dim3(gx, kComp, 1), blk, 0, st, Ct,
-                       ldc, n_orb, X, ldk, ldp, P, kpad, o);
+  } else if (mt == 2) {
+    hipLaunchKernelGGL((orbital_gemm_kernel<TI, TO, 2, WPB>),
+                       dim3(gx, kComp, (unsigned)((n_orb + 31) / 32)), blk, 0, st, Ct, ldc, n_orb,
+                       X, ldk, ldp, P, kpad, o);
   } else {
-    hipLaunchKernelGGL((orbital_gemm_kernel<TI, TO, 1, WPB>), dim3(gx, kComp, 1), blk, 0, st, Ct,
-                       ldc, n_orb, X, ldk, ldp, P, kpad, o);
+    hipLaunchKernelGGL((orbital_gemm_kernel<TI, TO, 1, WPB>),
+                       dim3(gx, kComp, (unsigned)((n_orb + 15) / 16)), blk, 0, st, Ct, ldc, n_orb,
+                       X, ldk, ldp, P, kpad, o);
   }
 }
 
-// wpb: waves (16-point tiles) per workgroup, 4 or 2.  Every output element
-// accumulates the same k-chunks in the same order either way, so the choice
-// changes speed only, never a result bit.
+// wpb: waves (16-point tiles) per workgroup, 4 or 2; mt: 16-orbital tiles
+// per wave (0: gemm_orbital_tiles' rule).  Every output element accumulates
+// the same k-chunks in the same order whatever the choice, so it changes
+// speed only, never a result bit.
 template <typename TI, typename TO>
 hipError_t launch_orbital_gemm(const TI *Ct, int ldc, int n_orb, const TI *X, int ldk, long ldp,
-                               long P, int kpad, const GemmOut &o, hipStream_t st, int wpb) {
+                               long P, int kpad, const GemmOut &o, hipStream_t st, int wpb,
+                               int mt) {
   if (P <= 0 || n_orb <= 0) return hipSuccess;
+  if (mt == 0) mt = gemm_orbital_tiles(n_orb, P);
+  if (!gemm_orbital_tiles_valid(mt, n_orb, ldc)) return hipErrorInvalidValue;
   if (wpb == 2)
-    gemm_grid<TI, TO, 2>(Ct, ldc, n_orb, X, ldk, ldp, P, kpad, o, st);
+    gemm_grid<TI, TO, 2>(Ct, ldc, n_orb, X, ldk, ldp, P, kpad, o, st, mt);
   else
-    gemm_grid<TI, TO, 4>(Ct, ldc, n_orb, X, ldk, ldp, P, kpad, o, st);
+    gemm_grid<TI, TO, 4>(Ct, ldc, n_orb, X, ldk, ldp, P, kpad, o, st, mt);
   return hipGetLastError();
 }
 
 template hipError_t launch_orbital_gemm<double, double>(const double *, int, int, const double *,
                                                         int, long, long, int, const GemmOut &,
-                                                        hipStream_t, int);
+                                                        hipStream_t, int, int);
 template hipError_t launch_orbital_gemm<float, float>(const float *, int, int, const float *, int,
                                                       long, long, int, const GemmOut &,
-                                                      hipStream_t, int);
+                                                      hipStream_t, int, int);
 template hipError_t launch_orbital_gemm<double, float>(const double *, int, int, const double *,
                                                        int, long, long, int, const GemmOut &,
-                                                       hipStream_t, int);
+                                                       hipStream_t, int, int);
 template hipError_t launch_orbital_gemm<float, bf16s>(const float *, int, int, const float *, int,
                                                       long, long, int, const GemmOut &,
-                                                      hipStream_t, int);
+                                                      hipStream_t, int, int);
 template hipError_t launch_orbital_gemm<__hip_bfloat16, float>(const __hip_bfloat16 *, int, int,
                                                                const __hip_bfloat16 *, int, long,
                                                                long, int, const GemmOut &,
-                                                               hipStream_t, int);
+                                                               hipStream_t, int, int);
 
 }  // namespace qmcb

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -76,9 +77,11 @@ struct qmcb_ctx {
   size_t dev_bytes = 0;  // device memory this context holds (arena + own allocations)
   char *arena = nullptr;
   size_t arena_cap = 0, arena_used = 0;
-  // single-electron orbital GEMM: waves per workgroup chosen by tune_gemm,
-  // per (spin, batch rows / 16)
-  std::map<std::pair<int, int>, int> gemm_wpb;
+  // single-electron orbital GEMM: (waves per workgroup, orbital tiles per
+  // wave) chosen by tune_gemm, per (spin, batch rows / 16); gemm_force: the
+  // pair set by QMCB_GEMM_TILE="wpb,mt" at creation ({0, 0}: tuned)
+  std::map<std::pair<int, int>, std::pair<int, int>> gemm_tile;
+  std::pair<int, int> gemm_force{0, 0};
   size_t ts = 8, tx = 8, to = 8;  // bytes: derivative storage, GEMM operand, GEMM output
 
   // resident walker state
@@ -374,24 +377,39 @@ MdetArgs mdet_args(qmcb_ctx *ctx, int m, int mode, const qmcb_results &out) {
 // The single-electron orbital GEMM has one row per walker, so its grid is
 // small (num / 64 workgroups per component) and how evenly it spreads over
 // the CUs decides its speed.  The first move of a spin at a batch size (per
-// 16 rows; walker groups run at their own sizes) times the 4-wave and 2-wave
-// workgroup variants on the real operands, three launches each, and keeps
-// the faster; batches below 4096 rows (where the timing runs would cost more
-// than they can save) take the 4-wave variant.  Both variants produce
+// 16 rows; walker groups run at their own sizes) times the tiles on the real
+// operands, three launches each, and keeps the fastest: workgroups of 4 and
+// 2 waves, each with the rule's orbital tile per wave (gemm_orbital_tiles)
+// and the half and double of it where they fit the padded orbitals.
+// Batches below 4096 rows (where the timing runs would cost more than they
+// can save) take 4 waves and the rule's tile.  Every candidate produces
 // identical bits.
 template <typename F>
 qmcb_status tune_gemm(qmcb_ctx *ctx, int sp, int num, F &&gemm) {
   const std::pair<int, int> key(sp, num >> 4);
-  if (num < 4096) {  // small batches: the 4-wave variant, no timing runs
-    ctx->gemm_wpb[key] = 4;
+  const int n_orb = ctx->nact[sp], ldc = ctx->npad[sp];
+  const int rule = gemm_orbital_tiles(n_orb, num);
+  if (ctx->gemm_force.first) {
+    if (!gemm_orbital_tiles_valid(ctx->gemm_force.second, n_orb, ldc)) {
+      ctx->err = "QMCB_GEMM_TILE: orbital tiles do not fit this spin's orbitals";
+      return QMCB_INVALID_ARG;
+    }
+    ctx->gemm_tile[key] = ctx->gemm_force;
     return QMCB_OK;
   }
+  if (num < 4096) {  // small batches: the rule's tile, no timing runs
+    ctx->gemm_tile[key] = {4, rule};
+    return QMCB_OK;
+  }
+  std::vector<std::pair<int, int>> cand;
+  for (int wpb : {4, 2})
+    for (int mt : {rule, rule / 2, rule * 2})
+      if (mt >= 1 && gemm_orbital_tiles_valid(mt, n_orb, ldc)) cand.push_back({wpb, mt});
   hipEvent_t e0, e1;
   QCHK(hipEventCreate(&e0));
   QCHK(hipEventCreate(&e1));
-  const int cand[2] = {4, 2};
-  float best_ms[2] = {1e30f, 1e30f};
-  for (int c = 0; c < 2; c++)
+  std::vector<float> best_ms(cand.size(), 1e30f);
+  for (size_t c = 0; c < cand.size(); c++)
     for (int rep = 0; rep < 3; rep++) {
       QCHK(hipEventRecord(e0, ctx->stream));
       QCHK(gemm(cand[c]));
@@ -403,7 +421,10 @@ qmcb_status tune_gemm(qmcb_ctx *ctx, int sp, int num, F &&gemm) {
     }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  ctx->gemm_wpb[key] = best_ms[1] < best_ms[0] ? cand[1] : cand[0];
+  size_t best = 0;  // ties keep the earlier candidate (4 waves, the rule's tile first)
+  for (size_t c = 1; c < cand.size(); c++)
+    if (best_ms[c] < best_ms[best]) best = c;
+  ctx->gemm_tile[key] = cand[best];
   return QMCB_OK;
 }
 
@@ -513,16 +534,16 @@ struct Engine {
     const int n = ctx->n[sp];
     GemmOut go{ctx->vrow, (long)ctx->npad[sp], 1, (long)kComp * ctx->npad[sp], 0, 1, nullptr,
                nullptr, 0, 0, 0};
-    auto gemm = [&](int wpb) {
+    auto gemm = [&](std::pair<int, int> tile) {
       return launch_orbital_gemm<TX, TO>((const TX *)ctx->Ct[sp], ctx->npad[sp], ctx->nact[sp],
                                          (const TX *)ctx->X, ctx->kpad, ctx->ldp, (long)num,
-                                         ctx->kpad, go, ctx->stream, wpb);
+                                         ctx->kpad, go, ctx->stream, tile.first, tile.second);
     };
-    auto it = ctx->gemm_wpb.find(std::make_pair(sp, num >> 4));
-    if (it == ctx->gemm_wpb.end()) {
+    auto it = ctx->gemm_tile.find(std::make_pair(sp, num >> 4));
+    if (it == ctx->gemm_tile.end()) {
       qmcb_status ts = tune_gemm(ctx, sp, num, gemm);
       if (ts != QMCB_OK) return ts;
-      it = ctx->gemm_wpb.find(std::make_pair(sp, num >> 4));
+      it = ctx->gemm_tile.find(std::make_pair(sp, num >> 4));
     }
     TIMED("orbital_gemm", gemm(it->second));
     (void)n;
@@ -701,6 +722,15 @@ size_t plan_device_bytes(const qmcb_system *s, const qmcb_options *o);
 
 qmcb_status create_impl(const qmcb_system *s, const qmcb_options *o, qmcb_ctx *ctx) {
   ctx->opt = *o;
+  if (const char *gt = std::getenv("QMCB_GEMM_TILE")) {
+    int wpb = 0, mt = 0;
+    if (std::sscanf(gt, "%d,%d", &wpb, &mt) != 2 || (wpb != 2 && wpb != 4) ||
+        (mt != 1 && mt != 2 && mt != 4 && mt != 8)) {
+      ctx->err = "QMCB_GEMM_TILE must be \"wpb,mt\": wpb 2 or 4, mt 1, 2, 4 or 8";
+      return QMCB_INVALID_ARG;
+    }
+    ctx->gemm_force = {wpb, mt};
+  }
   if (ctx->opt.dt <= 0) ctx->opt.dt = 0.1;
   if (ctx->opt.umrigar_a <= 0) ctx->opt.umrigar_a = 0.5;
   ctx->N = s->n_up + s->n_dn;

@@ -102,7 +102,11 @@ hipError_t launch_basis(const DevSystem &sys, const PointSet &pts, TS *X, int ld
 template <typename TI, typename TO>
 hipError_t launch_orbital_gemm(const TI *Ct, int ldc, int n_orb, const TI *X, int ldk, long ldp,
                                long P, int kpad, const GemmOut &o, hipStream_t st,
-                               int wpb = 4);
+                               int wpb = 4, int mt = 0);
+// orbital tiles per wave (16 orbitals each) chosen by rule for P rows, and
+// whether mt tiles keep every tile inside ldc padded orbitals
+int gemm_orbital_tiles(int n_orb, long P);
+bool gemm_orbital_tiles_valid(int mt, int n_orb, int ldc);
 
 // ---- kernel (c): Slater inverse, SM updates, traces ----
 struct SlaterArgs {
