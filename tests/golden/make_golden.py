@@ -6,7 +6,9 @@ stores inputs (seeds, system digest, configurations/moves) and outputs as
 small .npz fixtures beside this script.  Run in the authoring container only
 (needs /root/reference):
 
-    make -C oracle ref && python tests/golden/make_golden.py
+    make -C oracle ref && python tests/golden/make_golden.py [case ...]
+
+(no case names: every case)
 """
 from __future__ import annotations
 
@@ -94,13 +96,16 @@ CASES = [
 def main():
     if not os.path.exists(DRIVER):
         sys.exit("build the reference driver first: make -C oracle ref")
+    only = set(sys.argv[1:])
     with tempfile.TemporaryDirectory() as tmp:
         for name, cfg, nw, wseed, nsw, nmov in CASES:
+            if only and name not in only:
+                continue
             s = synth.make_system(cfg)
             R = synth.initial_walkers(s, nw, wseed)
             ev = run_eval(s, R, tmp)
             extra = {}
-            if cfg in (0, 1):
+            if cfg in (0, 1, 2):
                 ev_u = run_eval(s, R[: min(nw, 8)], tmp, drift="umrigar93_unequalelectrons", dt=0.1)
                 extra = {"umr_" + k: v for k, v in ev_u.items()}
             R0 = R[:nsw]
