@@ -89,7 +89,7 @@ CASES = [
     # name, config, n_walkers (eval), walker seed, n_seq_walkers, n_moves
     ("cfg0_h2", 0, 128, 101, 8, 40),
     ("cfg1_organic64", 1, 24, 102, 2, 70),
-    ("cfg2_mol256", 2, 4, 103, 1, 20),
+    ("cfg2_mol256", 2, 4, 103, 1, 136),  # moves 128 up and 8 down electrons
 ]
 
 
@@ -115,7 +115,7 @@ def main():
                 os.path.join(OUT, name + ".npz"),
                 config=cfg, walker_seed=wseed, n_walkers=nw, digest=s.digest(),
                 R=R, moves=mv, **{"ev_" + k: v for k, v in ev.items()},
-                **{"sq_" + k: v for k, v in sq.items()}, **extra)
+                **{"sq_" + k: v for k, v in sq.items() if k != "mod_drift"}, **extra)
             print(name, "written; E_L[0] =", ev["e_local"][0])
 
 
