@@ -679,9 +679,13 @@ __host__ __device__ constexpr int sm_unroll() {
                   : (sizeof(TS) == 4 ? QMCB_SM_UNROLL_F32
                                      : (sizeof(TS) == 2 ? QMCB_SM_UNROLL_BF16 : QMCB_SM_UNROLL_F64));
 }
-#ifndef QMCB_SM_MINWAVES
-#define QMCB_SM_MINWAVES 1
-#endif
+// waves per SIMD the register allocation must allow: four everywhere (at
+// most 128 VGPRs, no spills), two for the FP64 128-column tile, whose deeper
+// unroll holds about 200
+template <typename TS, int NC>
+__host__ __device__ constexpr int sm_waves() {
+  return (NC == 128 && sizeof(TS) == 8) ? 2 : 4;
+}
 #ifndef QMCB_SM_NT
 #define QMCB_SM_NT 1
 #endif
@@ -724,7 +728,8 @@ __device__ __forceinline__ bf16s ld_stream<bf16s>(const bf16s *p) {
 }
 
 template <typename TS, typename TO, int NC, int NT>
-__global__ void __launch_bounds__(NT, QMCB_SM_MINWAVES)
+__global__ void
+__attribute__((amdgpu_flat_work_group_size(NT, NT), amdgpu_waves_per_eu(sm_waves<TS, NC>())))
 sm_move_kernel(MoveArgs A, const int *walkers, int num) {
   constexpr int SL = NT / NC;
   constexpr int NW = NT / 64;
@@ -873,7 +878,9 @@ sm_move_kernel(MoveArgs A, const int *walkers, int num) {
       ld4(off, x[u], y[u], z[u], l[u]);
     }
   };
-  const bool stream = !flush && c < n;
+  // every move streams the rows through the same register pipeline; a flush
+  // (8 pending terms) also writes each folded row of A^-1 back
+  const bool stream = c < n;
   if (stream && o_beg < o_end) load(o_beg, ca, cx, cy, cz, cl);
   if (t < n) {
     double x = xr;
@@ -897,27 +904,22 @@ sm_move_kernel(MoveArgs A, const int *walkers, int num) {
       acc[5] += x * cr; acc[6] += y * cr; acc[7] += z * cr; acc[8] += l * cr;
       return a;
     };
-    if (stream) {
-      // read-only stream (its first U rows are already requested)
-      for (int o0 = o_beg; o0 < o_end; o0 += U) {
-        if (o0 + U < o_end) load(o0 + U, na, nx, ny, nz, nl);
+    // the stream (its first U rows are already requested); the loads of the
+    // next U rows are issued before this batch's rows are folded and, on a
+    // flush, stored, so the stores never wait on them
+    double *ip = inv + c;
+    for (int o0 = o_beg; o0 < o_end; o0 += U) {
+      if (o0 + U < o_end) load(o0 + U, na, nx, ny, nz, nl);
 #pragma unroll
-        for (int u = 0; u < U; u++)
-          if (o0 + u < o_end)
-            row(o0 + u, ca[u], (double)cx[u], (double)cy[u], (double)cz[u], (double)cl[u]);
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-          ca[u] = na[u]; cx[u] = nx[u]; cy[u] = ny[u]; cz[u] = nz[u]; cl[u] = nl[u];
+      for (int u = 0; u < U; u++)
+        if (o0 + u < o_end) {
+          const double a =
+              row(o0 + u, ca[u], (double)cx[u], (double)cy[u], (double)cz[u], (double)cl[u]);
+          if (flush) ip[(long)(o0 + u) * n] = a;
         }
-      }
-    } else {
-      // the pending terms are folded into the stored inverse on the way
-      double *ip = inv + c;
-      for (int o = o_beg; o < o_end; o++) {
-        const long off = (long)o * n;
-        TS x, y, z, l;
-        ld4(off, x, y, z, l);
-        ip[off] = row(o, ld_stream(ip + off), (double)x, (double)y, (double)z, (double)l);
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        ca[u] = na[u]; cx[u] = nx[u]; cy[u] = ny[u]; cz[u] = nz[u]; cl[u] = nl[u];
       }
     }
   }
