@@ -727,6 +727,23 @@ __device__ __forceinline__ bf16s ld_stream<bf16s>(const bf16s *p) {
   return v;
 }
 
+// the four derivative components of one element, adjacent in the
+// interleaved layout, with one store (two for FP64)
+__device__ __forceinline__ void st4(float *p, double x, double y, double z, double l) {
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  *reinterpret_cast<f4 *>(p) = f4{(float)x, (float)y, (float)z, (float)l};
+}
+__device__ __forceinline__ void st4(double *p, double x, double y, double z, double l) {
+  typedef double d2v __attribute__((ext_vector_type(2)));
+  reinterpret_cast<d2v *>(p)[0] = d2v{x, y};
+  reinterpret_cast<d2v *>(p)[1] = d2v{z, l};
+}
+__device__ __forceinline__ void st4(bf16s *p, double x, double y, double z, double l) {
+  const uint64_t q = (uint64_t)bf16s(x).b | ((uint64_t)bf16s(y).b << 16) |
+                     ((uint64_t)bf16s(z).b << 32) | ((uint64_t)bf16s(l).b << 48);
+  *reinterpret_cast<uint64_t *>(p) = q;
+}
+
 template <typename TS, typename TO, int NC, int NT>
 __global__ void
 __attribute__((amdgpu_flat_work_group_size(NT, NT), amdgpu_waves_per_eu(sm_waves<TS, NC>())))
@@ -1161,12 +1178,14 @@ sm_move_kernel(MoveArgs A, const int *walkers, int num) {
       double *g = A.s.gradPR + ((long)w * n + c) * 3;
       g[0] = gq[0][c]; g[1] = gq[1][c]; g[2] = gq[2][c];
     }
+    // column r of the value and derivative matrices: one thread per row o
+    // writes the element's four derivative components with one vector store
+    // (they are adjacent), not four threads with a scalar store each
     TS *dmw = (TS *)A.s.dmat + (long)w * kDer * nn2;
     double *dvw = A.s.dval + (long)w * nn2;
-    for (int i = t; i < kComp * n; i += NT) {
-      const int k = i / n, o = i % n;
-      if (k == 0) dvw[(long)o * n + r] = v[0][o];
-      else dmw[dm_at<TS>(k - 1, (long)o * n + r, nn2)] = (TS)v[k][o];
+    for (int o = t; o < n; o += NT) {
+      dvw[(long)o * n + r] = v[0][o];
+      st4(dmw + dm_at<TS>(0, (long)o * n + r, nn2), v[1][o], v[2][o], v[3][o], v[4][o]);
     }
     double *Uxw = A.Ux + (long)w * N * 3;
     if (A.jastrow_fused) {
