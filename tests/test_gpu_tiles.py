@@ -6,7 +6,8 @@ workgroup size (4 or 2 waves) and the orbital tile per wave (16, 32, 64 or
 in the same order whatever the tile, so the choice must not change a result
 bit.  QMCB_GEMM_TILE fixes the tile; each valid tile is run on the same
 walkers: a single-electron proposal (staged evaluate) and a few device VMC
-steps, compared bit for bit.
+steps, compared bit for bit, in every GEMM operand type (FP64, FP32, BF16
+with its orbital-major coefficients) and with BF16 derivative storage.
 """
 import numpy as np
 import pytest
@@ -35,7 +36,7 @@ def run(s, W, precision, tile, monkeypatch):
     (-1, 100, ["4,4", "2,4", "4,2", "4,8", "2,8"]),  # 50 per spin, padded to 64: no 8
     (2, None, ["4,4", "2,4", "4,2", "4,8", "2,8"]),  # 128 per spin
 ])
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16", "bf16store"])
 def test_gemm_tiles_bit_identical(config, n_elec, tiles, precision, monkeypatch):
     s = synth.make_system(config, seed=7, n_elec=n_elec) if config < 0 else synth.make_system(config)
     n = s.n_up
