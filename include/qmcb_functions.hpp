@@ -37,6 +37,8 @@ struct WalkerData {
   double lapPsiRatio = 0;
   double detRatio = 1;
   double U = 0;
+  double modificationRatio = 1;  // |modifiedGradPsiRatio| / |gradPsiRatio|
+  double x2 = 0, y2 = 0, z2 = 0;  // sums over the electrons of the squared coordinates
   bool singular = false;
   std::vector<double> gradPsiRatio;          // [N][3]
   std::vector<double> modifiedGradPsiRatio;  // [N][3]
@@ -91,6 +93,8 @@ class QMCFunctions {
       d.lapPsiRatio = lap_[i];
       d.detRatio = dr_[i];
       d.U = u_[i];
+      d.modificationRatio = mr_[i];
+      d.x2 = r2_[3 * i]; d.y2 = r2_[3 * i + 1]; d.z2 = r2_[3 * i + 2];
       d.singular = sing_[i] != 0;
       d.gradPsiRatio.assign(grad_.begin() + (size_t)i * N_ * 3, grad_.begin() + (size_t)(i + 1) * N_ * 3);
       d.modifiedGradPsiRatio.assign(mgrad_.begin() + (size_t)i * N_ * 3,
@@ -132,7 +136,8 @@ class QMCFunctions {
   }
   void alloc_(int num) {
     el_.resize(num); ke_.resize(num); pe_.resize(num); vne_.resize(num); vee_.resize(num);
-    lp_.resize(num); lap_.resize(num); dr_.resize(num); u_.resize(num);
+    lp_.resize(num); lap_.resize(num); dr_.resize(num); u_.resize(num); mr_.resize(num);
+    r2_.resize((size_t)num * 3);
     sg_.resize(num); sing_.resize(num);
     grad_.resize((size_t)num * N_ * 3);
     mgrad_.resize((size_t)num * N_ * 3);
@@ -143,13 +148,14 @@ class QMCFunctions {
     r.mod_drift = mgrad_.data(); r.lap_ratio = lap_.data(); r.e_local = el_.data();
     r.kinetic = ke_.data(); r.potential = pe_.data(); r.v_ne = vne_.data(); r.v_ee = vee_.data();
     r.det_ratio = dr_.data(); r.jastrow_u = u_.data(); r.singular = sing_.data();
+    r.mod_ratio = mr_.data(); r.r2 = r2_.data();
     return r;
   }
 
   qmcb_ctx *ctx_ = nullptr;
   int N_;
   std::vector<int32_t> ids_, staged_;
-  std::vector<double> R_, el_, ke_, pe_, vne_, vee_, lp_, lap_, dr_, u_, grad_, mgrad_;
+  std::vector<double> R_, el_, ke_, pe_, vne_, vee_, lp_, lap_, dr_, u_, mr_, r2_, grad_, mgrad_;
   std::vector<int8_t> sg_, sing_;
 };
 

@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define QMCB_ABI_VERSION 2
+#define QMCB_ABI_VERSION 3
 #define QMCB_PADE_MAX 8
 
 typedef enum qmcb_status {
@@ -184,6 +184,14 @@ typedef struct qmcb_results {
                                (single-electron evaluate); 1 for all-electron  */
   double *jastrow_u;        /* [num] U                                         */
   int8_t *singular;         /* [num] numerical failure flag                    */
+  double *mod_ratio;        /* [num] modificationRatio: |modified drift| /
+                               |gradPsiRatio| over all electrons (1 without a
+                               drift modification; QMCSCFJastrow.cpp:450-458).
+                               The device-buffer entry points need grad_lnpsi
+                               and mod_drift with it                          */
+  double *r2;               /* [num][3] x2, y2, z2: sums over the electrons of
+                               the squared coordinates of the evaluated
+                               configuration (QMCSCFJastrow.cpp:164-169)      */
 } qmcb_results;
 
 typedef struct qmcb_ctx qmcb_ctx;

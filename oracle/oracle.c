@@ -526,6 +526,7 @@ int orc_evaluate(const qmcb_system *s, orc_walker *w, const double *R, int which
   double *mgrad = malloc(sizeof(double) * N * 3);
   double lapR = 0.0;
   double logpsi = 0.0;
+  double mod_ratio = 0.0;
   int sign = 1;
   if (!sing) {
     /* calculate_Psi_quantities + update_SCF (QMCSCFJastrow.cpp:196-327), one
@@ -577,6 +578,7 @@ int orc_evaluate(const qmcb_system *s, orc_walker *w, const double *R, int which
     double lm = 0.0, lu = 0.0;
     for (int i = 0; i < 3 * N; i++) { lm += mgrad[i] * mgrad[i]; lu += grad[i] * grad[i]; }
     lm = sqrt(lm); lu = sqrt(lu);
+    mod_ratio = lm / lu; /* modificationRatio (QMCSCFJastrow.cpp:450-458) */
     if (isnan(lm) || isnan(lu) || isnan(lm / lu) || lu == 0) sing = 1;
   }
   if (out) {
@@ -593,6 +595,13 @@ int orc_evaluate(const qmcb_system *s, orc_walker *w, const double *R, int which
     if (out->det_ratio) out->det_ratio[0] = det_ratio;
     if (out->jastrow_u) out->jastrow_u[0] = w->U;
     if (out->singular) out->singular[0] = (int8_t)sing;
+    if (out->mod_ratio) out->mod_ratio[0] = mod_ratio;
+    if (out->r2) {
+      /* x2, y2, z2 of the evaluated configuration (QMCSCFJastrow.cpp:164-169) */
+      out->r2[0] = out->r2[1] = out->r2[2] = 0.0;
+      for (int e = 0; e < N; e++)
+        for (int k = 0; k < 3; k++) out->r2[k] += w->R[3 * e + k] * w->R[3 * e + k];
+    }
   }
   free(grad); free(mgrad);
   return sing;

@@ -54,7 +54,8 @@ class Results(C.Structure):
     _fields_ = [("log_psi", _dp), ("sign_psi", _i8p), ("grad_lnpsi", _dp),
                 ("mod_drift", _dp), ("lap_ratio", _dp), ("e_local", _dp),
                 ("kinetic", _dp), ("potential", _dp), ("v_ne", _dp), ("v_ee", _dp),
-                ("det_ratio", _dp), ("jastrow_u", _dp), ("singular", _i8p)]
+                ("det_ratio", _dp), ("jastrow_u", _dp), ("singular", _i8p),
+                ("mod_ratio", _dp), ("r2", _dp)]
 
 
 class CuspOrbital(C.Structure):
@@ -149,7 +150,7 @@ class ResultArrays:
 
     FIELDS = ("log_psi", "sign_psi", "grad_lnpsi", "mod_drift", "lap_ratio",
               "e_local", "kinetic", "potential", "v_ne", "v_ee", "det_ratio",
-              "jastrow_u", "singular")
+              "jastrow_u", "singular", "mod_ratio", "r2")
 
     def __init__(self, num: int, N: int):
         self.num, self.N = num, N
@@ -159,6 +160,8 @@ class ResultArrays:
                 self.a[f] = np.zeros(num, dtype=np.int8)
             elif f in ("grad_lnpsi", "mod_drift"):
                 self.a[f] = np.zeros((num, N, 3))
+            elif f == "r2":
+                self.a[f] = np.zeros((num, 3))
             else:
                 self.a[f] = np.zeros(num)
         r = Results()

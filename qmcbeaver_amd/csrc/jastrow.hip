@@ -280,6 +280,43 @@ hipError_t launch_assemble(const AssembleArgs &a, const int *walkers, int num, h
   return hipGetLastError();
 }
 
+// modificationRatio and x2/y2/z2 of an evaluate (QMCSCFJastrow.cpp:164-169,
+// 450-458), one wave per row, from the drift outputs already written and the
+// evaluated configuration
+__global__ void __launch_bounds__(64)
+results_extra_kernel(qmcb_results out, const double *R, const int *walkers, const double *propR,
+                     int whichE, int N, int num) {
+  const int b = blockIdx.x;
+  if (b >= num) return;
+  const long w = walkers ? walkers[b] : b;
+  const double *Rw = R + w * N * 3;
+  double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // |F|^2, |F_mod|^2, x2, y2, z2
+  for (int e = threadIdx.x; e < N; e += 64) {
+    if (out.mod_ratio) {
+      const double *g = out.grad_lnpsi + ((long)b * N + e) * 3;
+      const double *f = out.mod_drift + ((long)b * N + e) * 3;
+      for (int k = 0; k < 3; k++) { s[0] += g[k] * g[k]; s[1] += f[k] * f[k]; }
+    }
+    const double *x = e == whichE ? propR + 3L * b : Rw + 3 * e;
+    for (int k = 0; k < 3; k++) s[2 + k] += x[k] * x[k];
+  }
+  for (int k = 0; k < 5; k++) s[k] = wave_sum(s[k]);
+  if (threadIdx.x == 0) {
+    if (out.mod_ratio) out.mod_ratio[b] = sqrt(s[1]) / sqrt(s[0]);
+    if (out.r2)
+      for (int k = 0; k < 3; k++) out.r2[3L * b + k] = s[2 + k];
+  }
+}
+
+hipError_t launch_results_extra(const qmcb_results &out, const double *R, const int *walkers,
+                                const double *propR, int whichE, int N, int num, hipStream_t st) {
+  if (num <= 0 || (!out.mod_ratio && !out.r2)) return hipSuccess;
+  if (out.mod_ratio && (!out.grad_lnpsi || !out.mod_drift)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(results_extra_kernel, dim3(num), dim3(64), 0, st, out, R, walkers, propR,
+                     whichE, N, num);
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // Importance-sampled proposal of electron `moved` (QMCWalker.cpp:343-398):
 //   x' = x + tau F + sqrt(tau) g,  g ~ N(0,1)^3,  F = grad ln Psi (drift)

@@ -1009,6 +1009,8 @@ qmcb_status create_impl(const qmcb_system *s, const qmcb_options *o, qmcb_ctx *c
   QCHK(dalloc(ctx, &r.det_ratio, W));
   QCHK(dalloc(ctx, &r.jastrow_u, W));
   QCHK(dalloc(ctx, &r.singular, W));
+  QCHK(dalloc(ctx, &r.mod_ratio, W));
+  QCHK(dalloc(ctx, &r.r2, W * 3));
   ctx->initialized.assign(ctx->Wc, 0);
   QCHK(hipStreamSynchronize(ctx->stream));
   return QMCB_OK;
@@ -1053,26 +1055,46 @@ qmcb_status copy_out(qmcb_ctx *ctx, int num, qmcb_results *out) {
   QCHK(d2h(ctx, out->det_ratio, d.det_ratio, num));
   QCHK(d2h(ctx, out->jastrow_u, d.jastrow_u, num));
   QCHK(d2h(ctx, out->singular, d.singular, num));
+  QCHK(d2h(ctx, out->mod_ratio, d.mod_ratio, num));
+  QCHK(d2h(ctx, out->r2, d.r2, num * 3));
   QCHK(hipStreamSynchronize(ctx->stream));
   return QMCB_OK;
 }
 
 // evaluate with R already on the device (batch rows) and ids on the device
+// modificationRatio and x2/y2/z2 once the drift outputs are written
+static qmcb_status results_extra(qmcb_ctx *ctx, int num, const int *d_ids, int whichE,
+                                 const qmcb_results &dout) {
+  if (dout.mod_ratio && (!dout.grad_lnpsi || !dout.mod_drift)) {
+    ctx->err = "mod_ratio needs the grad_lnpsi and mod_drift outputs";
+    return QMCB_INVALID_ARG;
+  }
+  QCHK(launch_results_extra(dout, ctx->R, d_ids, ctx->propR, whichE, ctx->N, num, ctx->stream));
+  return QMCB_OK;
+}
+
 qmcb_status evaluate_core(qmcb_ctx *ctx, int num, const int *d_ids, const double *d_R,
                           int whichE, const qmcb_results &dout) {
   if (whichE == -1) {
     QCHK(launch_scatter_rows(ctx->R, d_R, d_ids, num, ctx->N * 3, ctx->stream));
     qmcb_status s = DISPATCH(ctx, all_electron(ctx, d_ids, num));
     if (s != QMCB_OK) return s;
-    return DISPATCH(ctx, assemble(ctx, d_ids, num, dout));
+    s = DISPATCH(ctx, assemble(ctx, d_ids, num, dout));
+    if (s != QMCB_OK) return s;
+    return results_extra(ctx, num, d_ids, -1, dout);
   }
   qmcb_status s = ensure_staging(ctx);
   if (s != QMCB_OK) return s;
   s = DISPATCH(ctx, move_front(ctx, d_ids, num, whichE));
   if (s != QMCB_OK) return s;
-  if (ctx->multi) return DISPATCH(ctx, mdet_stage(ctx, d_ids, num, whichE, 0, dout, 0.0, nullptr));
-  MoveArgs a = DISPATCH(ctx, move_args(ctx, whichE, 0, dout));
-  return DISPATCH(ctx, sm(ctx, a, d_ids, num));
+  if (ctx->multi) {
+    s = DISPATCH(ctx, mdet_stage(ctx, d_ids, num, whichE, 0, dout, 0.0, nullptr));
+  } else {
+    MoveArgs a = DISPATCH(ctx, move_args(ctx, whichE, 0, dout));
+    s = DISPATCH(ctx, sm(ctx, a, d_ids, num));
+  }
+  if (s != QMCB_OK) return s;
+  return results_extra(ctx, num, d_ids, whichE, dout);
 }
 
 // The device memory a context of this system and these options holds in its
@@ -1183,6 +1205,8 @@ size_t plan_device_bytes(const qmcb_system *s, const qmcb_options *o) {
   add(W * N * 3, 8);
   for (int i = 0; i < 8; i++) add(W, 8);   // lap_ratio .. jastrow_u
   add(W, 1);                                 // singular
+  add(W, 8);                                 // mod_ratio
+  add(W * 3, 8);                             // r2
   // staging (first single-electron evaluate or move)
   const long nm = std::max(n[0], n[1]);
   add(WD * nm * nm, 8);
@@ -1362,7 +1386,9 @@ qmcb_status qmcb_evaluate_device_noalloc(qmcb_ctx *ctx, int32_t num, const int32
   QCHK(launch_zero_pad_rows(ctx->scratch_dev, ctx->tx, ctx->B, ctx->kpad, cap, ctx->stream));
   qmcb_status s = DISPATCH(ctx, all_electron(ctx, ids_dev, num, ctx->scratch_dev, cap));
   if (s != QMCB_OK) return s;
-  return DISPATCH(ctx, assemble(ctx, ids_dev, num, *out_dev));
+  s = DISPATCH(ctx, assemble(ctx, ids_dev, num, *out_dev));
+  if (s != QMCB_OK) return s;
+  return results_extra(ctx, num, ids_dev, -1, *out_dev);
 }
 
 qmcb_status qmcb_accept(qmcb_ctx *ctx, int32_t num, const int32_t *ids, const uint8_t *mask) {

@@ -287,6 +287,11 @@ struct AssembleArgs {
   int *age;              // [Wc] walker age, reset for a fresh walker (may be null)
 };
 hipError_t launch_assemble(const AssembleArgs &a, const int *walkers, int num, hipStream_t st);
+// out.mod_ratio and out.r2 of rows [0, num) from out.grad_lnpsi / out.mod_drift
+// and the configuration: resident R of walkers[b] with electron whichE (>= 0)
+// at propR[b]
+hipError_t launch_results_extra(const qmcb_results &out, const double *R, const int *walkers,
+                                const double *propR, int whichE, int N, int num, hipStream_t st);
 
 // proposal of a single-electron importance-sampled move (QMCWalker.cpp:343-398)
 struct ProposeArgs {

@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
     L = q.lib()
     for f in declared_functions():
         assert hasattr(L, f), f
-    assert L.qmcb_abi_version() == 2
+    assert L.qmcb_abi_version() == 3
 
 
 def test_create_rejects_bad_arguments():

@@ -50,6 +50,8 @@ def test_all_electron_fp64_matches_reference(golden):
         assert rel(res[k], g["ev_" + k]) < 1e-10, (k, rel(res[k], g["ev_" + k]))
     assert rel(res["grad_lnpsi"], g["ev_grad_lnpsi"]) < 1e-10
     assert rel(res["mod_drift"], g["ev_mod_drift"]) < 1e-10
+    assert rel(res["mod_ratio"], g["ev_mod_ratio"]) < 1e-10
+    np.testing.assert_allclose(res["r2"], (R ** 2).sum(axis=1), rtol=1e-14)
     sign_ref = np.sign(g["ev_detA"] * g["ev_detB"])
     assert np.array_equal(res["sign_psi"], sign_ref.astype(np.int8))
 
@@ -77,6 +79,7 @@ def test_umrigar93_drift_matches_reference(golden):
     with Context(s.pack(), precision="fp64", max_walkers=n, drift_mod=1, dt=0.1) as ctx:
         res = ctx.evaluate(np.arange(n), g["R"][:n])
     assert rel(res["mod_drift"], g["umr_mod_drift"]) < 1e-10
+    assert rel(res["mod_ratio"], g["umr_mod_ratio"]) < 1e-10
 
 
 def test_single_electron_sequence_matches_reference(golden):
@@ -100,6 +103,8 @@ def test_single_electron_sequence_matches_reference(golden):
             assert rel(res["log_psi"], g["sq_log_psi"][idx]) < 1e-9, m
             assert rel(res["e_local"], g["sq_e_local"][idx]) < 1e-9, m
             assert rel(res["grad_lnpsi"], g["sq_grad_lnpsi"][idx]) < 1e-9, m
+            assert rel(res["mod_ratio"], g["sq_mod_ratio"][idx]) < 1e-9, m
+            np.testing.assert_allclose(res["r2"], (R ** 2).sum(axis=1), rtol=1e-14)
             ctx.accept(ids, np.ones(nw, dtype=np.uint8))
 
 

@@ -43,6 +43,8 @@ def test_golden_all_electron(golden):
         assert rel(res[k], g["ev_" + k]) < 1e-10, k
     assert rel(res["grad_lnpsi"], g["ev_grad_lnpsi"]) < 1e-10
     assert rel(res["mod_drift"], g["ev_mod_drift"]) < 1e-10
+    assert rel(res["mod_ratio"], g["ev_mod_ratio"]) < 1e-12
+    np.testing.assert_allclose(res["r2"], (g["R"] ** 2).sum(axis=1), rtol=1e-14)
     assert not res["singular"].any()
 
 
@@ -55,6 +57,8 @@ def test_golden_umrigar93_drift(golden):
     res = o.evaluate_all(g["R"][:n])
     assert rel(res["mod_drift"], g["umr_mod_drift"]) < 1e-10
     assert rel(res["e_local"], g["umr_e_local"]) < 1e-10
+    assert rel(res["mod_ratio"], g["umr_mod_ratio"]) < 1e-10
+    assert np.all(g["umr_mod_ratio"] < 1.0)  # the modification shortens the drift
 
 
 def test_golden_single_electron_sequence(golden):

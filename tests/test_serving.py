@@ -65,8 +65,9 @@ def test_plan_resident_state_recount():
         per += 8 * N * 3 * 6                               # R Ux Rin pUx grad drift
         per += 8 * 3 * 2                                   # propR gauss
         # per-walker doubles: U, Uxx Vee Ven cur_el, unif, pU, pUxx pVee pVen,
-        # 4 energy accumulators, 3 observables, weight acc_w, ln|Psi|, 8 outputs
-        per += 8 * (1 + 4 + 1 + 1 + 3 + 4 + 3 + 2 + 1 + 8)
+        # 4 energy accumulators, 3 observables, weight acc_w, ln|Psi|, 8 outputs,
+        # mod_ratio, r2
+        per += 8 * (1 + 4 + 1 + 1 + 3 + 4 + 3 + 2 + 1 + 8 + 1 + 3)
         per += 4 * (1 + 2 + 1)                             # active, pairs, ids
         per += 4 * 2                                       # age, refresh map
         per += 1 + 1 + 1                                   # mask, output sign, output singular flag

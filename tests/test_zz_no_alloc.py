@@ -24,7 +24,8 @@ pytestmark = pytest.mark.gpu
 FIELDS = [("log_psi", "f8", 1), ("sign_psi", "i1", 1), ("grad_lnpsi", "f8", 3),
           ("mod_drift", "f8", 3), ("lap_ratio", "f8", 1), ("e_local", "f8", 1),
           ("kinetic", "f8", 1), ("potential", "f8", 1), ("v_ne", "f8", 1), ("v_ee", "f8", 1),
-          ("det_ratio", "f8", 1), ("jastrow_u", "f8", 1), ("singular", "i1", 1)]
+          ("det_ratio", "f8", 1), ("jastrow_u", "f8", 1), ("singular", "i1", 1),
+          ("mod_ratio", "f8", 1), ("r2", "f8", 0)]
 
 
 def say(*a):
@@ -77,7 +78,7 @@ def call(hip, ctx, entry, ids, R):
     d_R = hip.upload(np.asarray(R, np.float64))
     r, shapes = Results(), {}
     for name, kind, per in FIELDS:
-        shape = (num, N, 3) if per == 3 else (num,)
+        shape = (num, N, 3) if per == 3 else ((num, 3) if per == 0 else (num,))
         dt = np.float64 if kind == "f8" else np.int8
         p = hip.upload(np.full(shape, 7, dtype=dt))  # poisoned: every entry must be written
         shapes[name] = (p, shape, dt)
