@@ -205,7 +205,9 @@ int32_t qmcb_abi_version(void);
  * variable QMCB_GEMM_TILE="wpb,mt" (wpb 2 or 4 waves per workgroup, mt 1, 2,
  * 4 or 8 orbital tiles of 16 per wave), read here, fixes the single-electron
  * orbital GEMM's tile instead of the timing autotuner; every tile gives the
- * same bits. */
+ * same bits.  QMCB_SEPARATE_PROPOSE=1 makes the device VMC launch its
+ * proposal kernel every step instead of drawing the next proposal inside the
+ * move kernel (same bits). */
 qmcb_status qmcb_create(const qmcb_system *sys, const qmcb_options *opt,
                         qmcb_ctx **out);
 

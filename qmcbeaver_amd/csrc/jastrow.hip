@@ -333,34 +333,10 @@ __global__ void __launch_bounds__(256) propose_kernel(ProposeArgs A, const int *
   const double *gp = A.cgrad ? A.cgrad + ((long)w * A.N + m) * 3
                              : A.gradPR[sp] + ((long)w * nsp + le) * 3;
   const double *ux = A.Ux + ((long)w * A.N + m) * 3;
-  const double *x = A.R + ((long)w * A.N + m) * 3;
-  const uint32_t k0 = (uint32_t)A.seed, k1 = (uint32_t)(A.seed >> 32);
-  double u[8];
-  for (int c = 0; c < 2; c++) {
-    u4 ctr = {(uint32_t)A.step, (uint32_t)((uint64_t)A.step >> 32), (uint32_t)w, (uint32_t)c};
-    u4 r = philox4x32(ctr, k0, k1);
-    u[4 * c + 0] = u01(r.x, r.y);
-    u[4 * c + 1] = u01(r.y, r.z);
-    u[4 * c + 2] = u01(r.z, r.w);
-    u[4 * c + 3] = u01(r.w, r.x);
-  }
-  // Box-Muller on (u0,u1), (u2,u3)
-  const double two_pi = 6.283185307179586;
-  const double r1 = sqrt(-2.0 * log(u[0])), r2 = sqrt(-2.0 * log(u[2]));
-  const double g[3] = {r1 * cos(two_pi * u[1]), r1 * sin(two_pi * u[1]), r2 * cos(two_pi * u[3])};
-  const double st = sqrt(A.tau);
-  double v[3] = {gp[0] + ux[0], gp[1] + ux[1], gp[2] + ux[2]};
-  if (A.drift_mod == QMCB_DRIFT_UMRIGAR93_UNEQUAL) {
-    const double xm[3] = {x[0], x[1], x[2]};
-    const double f = drift_factor_unequal(v, xm, A.n_nuc, A.nuc_xyz, A.nuc_z, A.tau);
-    for (int k = 0; k < 3; k++) v[k] *= f;
-  }
-  for (int k = 0; k < 3; k++) {
-    const double chi = st * g[k];
-    A.gauss[3L * b + k] = chi;
-    A.propR[3L * b + k] = x[k] + A.tau * v[k] + chi;
-  }
-  A.unif[b] = u[4];
+  const double *xp = A.R + ((long)w * A.N + m) * 3;
+  const double v[3] = {gp[0] + ux[0], gp[1] + ux[1], gp[2] + ux[2]};
+  const double x[3] = {xp[0], xp[1], xp[2]};
+  propose_row(A, A.row_from_walker ? (long)w - A.row_base : (long)b, w, v, x);
 }
 
 hipError_t launch_propose(const ProposeArgs &a, const int *walkers, int num, hipStream_t st) {

@@ -82,6 +82,7 @@ struct qmcb_ctx {
   // pair set by QMCB_GEMM_TILE="wpb,mt" at creation ({0, 0}: tuned)
   std::map<std::pair<int, int>, std::pair<int, int>> gemm_tile;
   std::pair<int, int> gemm_force{0, 0};
+  bool separate_propose = false;  // QMCB_SEPARATE_PROPOSE set at creation
   size_t ts = 8, tx = 8, to = 8;  // bytes: derivative storage, GEMM operand, GEMM output
 
   // resident walker state
@@ -731,6 +732,7 @@ qmcb_status create_impl(const qmcb_system *s, const qmcb_options *o, qmcb_ctx *c
     }
     ctx->gemm_force = {wpb, mt};
   }
+  if (const char *sp = std::getenv("QMCB_SEPARATE_PROPOSE")) ctx->separate_propose = sp[0] == '1';
   if (ctx->opt.dt <= 0) ctx->opt.dt = 0.1;
   if (ctx->opt.umrigar_a <= 0) ctx->opt.umrigar_a = 0.5;
   ctx->N = s->n_up + s->n_dn;
@@ -1596,7 +1598,10 @@ static qmcb_status refresh_plan(qmcb_ctx *ctx, const int *slots, int num) {
 
 // The refresh share, after a move of spin sp, of the walkers at list
 // positions [first, last): the phase whose last move this is, if any.
-static qmcb_status staggered_refresh(qmcb_ctx *ctx, long first, long last, int sp) {
+// (*map, *cnt): the device map and count of the walkers re-inverted (0 if none)
+static qmcb_status staggered_refresh(qmcb_ctx *ctx, long first, long last, int sp,
+                                     const int **map = nullptr, int *cnt = nullptr) {
+  if (cnt) *cnt = 0;
   const int K = ctx->rplan_K, S = ctx->rplan_S;
   if (K <= 0) return QMCB_OK;
   const int c = (int)(ctx->moves_of_spin[sp] % K), q = c / S;
@@ -1604,7 +1609,10 @@ static qmcb_status staggered_refresh(qmcb_ctx *ctx, long first, long last, int s
   const int *p0 = ctx->rpos.data() + ctx->roff[q], *p1 = ctx->rpos.data() + ctx->roff[q + 1];
   const int *b = std::lower_bound(p0, p1, (int)first), *e = std::lower_bound(b, p1, (int)last);
   if (e == b) return QMCB_OK;
-  return DISPATCH(ctx, refresh(ctx, ctx->d_rmap + (b - ctx->rpos.data()), (int)(e - b), 1 << sp));
+  const int *d = ctx->d_rmap + (b - ctx->rpos.data());
+  if (map) *map = d;
+  if (cnt) *cnt = (int)(e - b);
+  return DISPATCH(ctx, refresh(ctx, d, (int)(e - b), 1 << sp));
 }
 
 // All-electron moves (opt.all_electron_moves).  ae_args fills the record and
@@ -1734,7 +1742,7 @@ qmcb_status qmcb_vmc_steps(qmcb_ctx *ctx, int32_t num, int64_t first_step, int32
   }
   const int K = ctx->opt.refresh_interval;
   qmcb_results none{};
-  auto propose = [&](long step, int m, const int *d_map, int cnt) -> qmcb_status {
+  auto propose_args = [&](long step, int m) {
     ProposeArgs pa{};
     pa.N = ctx->N;
     pa.moved = m;
@@ -1751,7 +1759,32 @@ qmcb_status qmcb_vmc_steps(qmcb_ctx *ctx, int32_t num, int64_t first_step, int32
     pa.propR = ctx->propR;
     pa.gauss = ctx->gauss;
     pa.unif = ctx->unif;
+    return pa;
+  };
+  auto propose = [&](long step, int m, const int *d_map, int cnt) -> qmcb_status {
+    const ProposeArgs pa = propose_args(step, m);
     TIMED("propose", launch_propose(pa, d_map, cnt, ctx->stream));
+    return QMCB_OK;
+  };
+  // One determinant: a step's sm_move also draws the next step's proposal
+  // (propose_next), so only a call's first step launches propose_kernel;
+  // walkers a re-inversion touched between the two steps are proposed again
+  // from their refreshed gradients (the same draws), which leaves every bit as
+  // the separate launch would.  QMCB_SEPARATE_PROPOSE (read at creation)
+  // keeps one propose launch per step.
+  const bool fuse_propose = !ctx->multi && !ctx->separate_propose;
+  auto next_proposal = [&](MoveArgs &a, long step, int s, int count) {
+    if (!fuse_propose || s + 1 >= count) return;
+    a.propose_next = 1;
+    a.next = propose_args(step + 1, (int)((step + 1) % ctx->N));
+  };
+  auto repropose = [&](long step, int s, int count, long first, const int *map, int cnt)
+      -> qmcb_status {
+    if (!fuse_propose || s + 1 >= count || cnt <= 0) return QMCB_OK;
+    ProposeArgs pa = propose_args(step + 1, (int)((step + 1) % ctx->N));
+    pa.row_from_walker = 1;
+    pa.row_base = (int)first;
+    TIMED("propose", launch_propose(pa, map, cnt, ctx->stream));
     return QMCB_OK;
   };
   // `count` steps from step0 with the walkers in G groups
@@ -1783,7 +1816,7 @@ qmcb_status qmcb_vmc_steps(qmcb_ctx *ctx, int32_t num, int64_t first_step, int32
           // group trails the one before it and its front overlaps that group's
           // Sherman-Morrison pass; group 0 runs free
           if (g > 0) QCHK(hipStreamWaitEvent(ctx->stream, ctx->gfront[g - 1], 0));
-          st = propose(step, m, d_map, (int)cnt);
+          if (s == 0 || !fuse_propose) st = propose(step, m, d_map, (int)cnt);
           if (st == QMCB_OK)
             st = DISPATCH(ctx, move_front(ctx, d_map, (int)cnt, m, !jastrow_fused(ctx, m)));
           if (st == QMCB_OK) QCHK(hipEventRecord(ctx->gfront[g], ctx->stream));
@@ -1792,9 +1825,13 @@ qmcb_status qmcb_vmc_steps(qmcb_ctx *ctx, int32_t num, int64_t first_step, int32
             a.tau = tau;
             a.jastrow_fused = jastrow_fused(ctx, m);
             if (m == 0) a.obs = ctx->d_obs;
+            next_proposal(a, step, s, count);
             st = DISPATCH(ctx, sm(ctx, a, d_map, (int)cnt));
           }
-          if (st == QMCB_OK) st = staggered_refresh(ctx, first, first + cnt, sp);
+          const int *rmap = nullptr;
+          int rcnt = 0;
+          if (st == QMCB_OK) st = staggered_refresh(ctx, first, first + cnt, sp, &rmap, &rcnt);
+          if (st == QMCB_OK) st = repropose(step, s, count, first, rmap, rcnt);
           if (st != QMCB_OK) return st;
         }
         ctx->moves_of_spin[sp]++;
@@ -1810,7 +1847,7 @@ qmcb_status qmcb_vmc_steps(qmcb_ctx *ctx, int32_t num, int64_t first_step, int32
         }
         continue;
       }
-      st = propose(step, m, nullptr, num);
+      if (s == 0 || !fuse_propose) st = propose(step, m, nullptr, num);
       if (st != QMCB_OK) return st;
       st = DISPATCH(ctx, move_front(ctx, nullptr, num, m, !jastrow_fused(ctx, m)));
       if (st != QMCB_OK) return st;
@@ -1826,13 +1863,17 @@ qmcb_status qmcb_vmc_steps(qmcb_ctx *ctx, int32_t num, int64_t first_step, int32
         a.tau = tau;
         a.jastrow_fused = jastrow_fused(ctx, m);
         if (m == 0) a.obs = ctx->d_obs;
+        next_proposal(a, step, s, count);
         st = DISPATCH(ctx, sm(ctx, a, nullptr, num));
       }
       if (st != QMCB_OK) return st;
       if (m == 0)
         TIMED("series", launch_series_reduce(ctx->d_obs, num,
                                              ctx->d_series + 4L * ctx->series_len++, ctx->stream));
-      st = staggered_refresh(ctx, 0, num, sp);
+      const int *rmap = nullptr;
+      int rcnt = 0;
+      st = staggered_refresh(ctx, 0, num, sp, &rmap, &rcnt);
+      if (st == QMCB_OK && !ctx->multi) st = repropose(step, s, count, 0, rmap, rcnt);
       if (st != QMCB_OK) return st;
       ctx->moves_of_spin[sp]++;
     }

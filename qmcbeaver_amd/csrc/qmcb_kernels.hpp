@@ -147,6 +147,62 @@ hipError_t launch_traces_all(const SlaterArgs &a, const int *walkers, int num, h
 
 // Single-electron proposal: everything the evaluate contract needs at the
 // proposed position, plus optional in-kernel Metropolis and commit.
+struct ProposeArgs {
+  int N, moved, n_up;
+  const double *R;
+  const double *gradPR[2];
+  const double *cgrad;   // several determinants: [Wc][N][3] grad ln D of the sum (else null)
+  const double *Ux;
+  double tau;
+  // drift of the moved electron: gradPsiRatio (QMCB_DRIFT_NONE) or its
+  // umrigar93_unequalelectrons modification with time step tau
+  int drift_mod;
+  int n_nuc;
+  const double *nuc_xyz, *nuc_z;
+  uint64_t seed;
+  long step;
+  double *propR, *gauss, *unif;
+  // batch row of walker w: the launch's own index, or with row_from_walker
+  // w - row_base (a launch over a subset of a step's walkers)
+  int row_from_walker, row_base;
+};
+hipError_t launch_propose(const ProposeArgs &a, const int *walkers, int num, hipStream_t st);
+
+// The importance-sampled proposal of electron A.moved of walker w into batch
+// row `row` (QMCWalker.cpp:343-398): x' = x + tau F + sqrt(tau) g, with F the
+// electron's drift v (gradPsiRatio) or its umrigar93_unequalelectrons
+// modification, g ~ N(0,1)^3 by Box-Muller from Philox keyed by (seed, step,
+// walker), and the uniform of the Metropolis test from the same draw.
+__device__ inline void propose_row(const ProposeArgs &A, long row, int w, const double v_in[3],
+                                   const double x[3]) {
+  const uint32_t k0 = (uint32_t)A.seed, k1 = (uint32_t)(A.seed >> 32);
+  double u[8];
+  for (int c = 0; c < 2; c++) {
+    u4 ctr = {(uint32_t)A.step, (uint32_t)((uint64_t)A.step >> 32), (uint32_t)w, (uint32_t)c};
+    u4 r = philox4x32(ctr, k0, k1);
+    u[4 * c + 0] = u01(r.x, r.y);
+    u[4 * c + 1] = u01(r.y, r.z);
+    u[4 * c + 2] = u01(r.z, r.w);
+    u[4 * c + 3] = u01(r.w, r.x);
+  }
+  // Box-Muller on (u0,u1), (u2,u3)
+  const double two_pi = 6.283185307179586;
+  const double r1 = sqrt(-2.0 * log(u[0])), r2 = sqrt(-2.0 * log(u[2]));
+  const double g[3] = {r1 * cos(two_pi * u[1]), r1 * sin(two_pi * u[1]), r2 * cos(two_pi * u[3])};
+  const double st = sqrt(A.tau);
+  double v[3] = {v_in[0], v_in[1], v_in[2]};
+  if (A.drift_mod == QMCB_DRIFT_UMRIGAR93_UNEQUAL) {
+    const double f = drift_factor_unequal(v, x, A.n_nuc, A.nuc_xyz, A.nuc_z, A.tau);
+    for (int k = 0; k < 3; k++) v[k] *= f;
+  }
+  for (int k = 0; k < 3; k++) {
+    const double chi = st * g[k];
+    A.gauss[3L * row + k] = chi;
+    A.propR[3L * row + k] = x[k] + A.tau * v[k] + chi;
+  }
+  A.unif[row] = u[4];
+}
+
 struct MoveArgs {
   SlaterArgs s;          // moved spin
   SlaterArgs other;      // the other spin (read only)
@@ -192,6 +248,10 @@ struct MoveArgs {
   // observable sample of this step (null except once per sweep):
   // obs[3 b + {0,1,2}] = {p E_L(R') + (1 - p) E_L(R), weight, p}
   double *obs;
+  // VMC: the next step's proposal (electron next.moved, step next.step) of
+  // each walker from the state this move leaves, into the same batch row
+  int propose_next;
+  ProposeArgs next;
 };
 
 // One row {sum w E, sum w, sum w p, walkers} of the per-sweep observable
@@ -294,23 +354,7 @@ hipError_t launch_results_extra(const qmcb_results &out, const double *R, const 
                                 const double *propR, int whichE, int N, int num, hipStream_t st);
 
 // proposal of a single-electron importance-sampled move (QMCWalker.cpp:343-398)
-struct ProposeArgs {
-  int N, moved, n_up;
-  const double *R;
-  const double *gradPR[2];
-  const double *cgrad;   // several determinants: [Wc][N][3] grad ln D of the sum (else null)
-  const double *Ux;
-  double tau;
-  // drift of the moved electron: gradPsiRatio (QMCB_DRIFT_NONE) or its
-  // umrigar93_unequalelectrons modification with time step tau
-  int drift_mod;
-  int n_nuc;
-  const double *nuc_xyz, *nuc_z;
-  uint64_t seed;
-  long step;
-  double *propR, *gauss, *unif;
-};
-hipError_t launch_propose(const ProposeArgs &a, const int *walkers, int num, hipStream_t st);
+
 
 // all-electron moves (walk_ae.hip; QMCWalker.cpp:343-398, 756-780,
 // 1578-1685): the walker record at the current positions, the proposal

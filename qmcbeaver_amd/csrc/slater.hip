@@ -1252,6 +1252,27 @@ sm_move_kernel(MoveArgs A, const int *walkers, int num) {
       A.obs[3L * b + 1] = wt;
       A.obs[3L * b + 2] = p;
     }
+    if (A.propose_next) {
+      // the next step's proposal from the values this move leaves behind,
+      // the ones the next step's propose_kernel would read: electron m2's
+      // position, its ln D gradient (this spin's gq if accepted) and its
+      // Jastrow gradient (the proposal's if accepted)
+      const ProposeArgs &P = A.next;
+      const int m2 = P.moved;
+      const bool moved2 = m2 == A.moved;
+      double x[3], gd[3], ux[3], vd[3];
+      for (int k = 0; k < 3; k++) x[k] = (moved2 && accept) ? A.propR[3L * b + k] : Rw[3 * m2 + k];
+      if (m2 >= ss && m2 < ss + n) {
+        for (int k = 0; k < 3; k++) gd[k] = accept ? gq[k][m2 - ss] : gps[3 * (m2 - ss) + k];
+      } else {
+        for (int k = 0; k < 3; k++) gd[k] = gpo[3 * (m2 - so) + k];
+      }
+      for (int k = 0; k < 3; k++)
+        ux[k] = !accept ? cUx[3 * m2 + k]
+                        : (A.jastrow_fused ? (moved2 ? jsum(4 + k) : s_ux[m2][k]) : pUx[3 * m2 + k]);
+      for (int k = 0; k < 3; k++) vd[k] = gd[k] + ux[k];
+      propose_row(P, b, w, vd, x);
+    }
   }
 }
 

@@ -201,3 +201,31 @@ def test_vmc_refresh_is_staggered(W, K, S):
     assert kt["lu_inverse"]["launches"] == expect
     assert kt["traces_all"]["launches"] == expect
     assert kt["sm_move"]["launches"] == steps
+
+
+@pytest.mark.parametrize("config,precision,W,groups", [(1, "fp64", 256, 0), (1, "fp32", 1024, 2),
+                                                       (2, "fp32", 512, 0)])
+def test_vmc_next_proposal_in_sm_move_bit_identical(config, precision, W, groups, monkeypatch):
+    """A step's sm_move draws the next step's proposal (one propose launch per
+    call, walkers that a re-inversion touched proposed again); the walk equals
+    the one with a propose launch per step bit for bit, with the staggered
+    FP64 re-inversion and walker groups."""
+    s = synth.make_system(config)
+    R0 = synth.initial_walkers(s, W, 31)
+    print("case fused proposal config", config, precision, W, "groups", groups, flush=True)
+    out = []
+    for separate in ("0", "1"):
+        monkeypatch.setenv("QMCB_SEPARATE_PROPOSE", separate)
+        with Context(s.pack(), precision=precision, max_walkers=W, seed=17,
+                     refresh_interval=64 if precision != "fp64" else 0,
+                     walker_groups=groups) as c:
+            c.evaluate(np.arange(W), R0)
+            rec = []
+            for call in range(2):
+                st = c.vmc_steps(W, 150 * call, 150, 0.02)
+                rec.append((c.positions(W), c.sample_series(), st.sum_accept, st.sum_e_local))
+            out.append(rec)
+    for a, b in zip(*out):
+        np.testing.assert_array_equal(a[0], b[0])
+        np.testing.assert_array_equal(a[1], b[1])
+        assert a[2] == b[2] and a[3] == b[3]
