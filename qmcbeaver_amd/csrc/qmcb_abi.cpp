@@ -1770,9 +1770,11 @@ qmcb_status qmcb_vmc_steps(qmcb_ctx *ctx, int32_t num, int64_t first_step, int32
   // (propose_next), so only a call's first step launches propose_kernel;
   // walkers a re-inversion touched between the two steps are proposed again
   // from their refreshed gradients (the same draws), which leaves every bit as
-  // the separate launch would.  QMCB_SEPARATE_PROPOSE (read at creation)
-  // keeps one propose launch per step.
-  const bool fuse_propose = !ctx->multi && !ctx->separate_propose;
+  // the separate launch would.  Not with BF16 derivative storage, whose
+  // shorter move launch measured slower with the draw's serial tail;
+  // QMCB_SEPARATE_PROPOSE (read at creation) keeps one launch per step too.
+  const bool fuse_propose = !ctx->multi && !ctx->separate_propose &&
+                            ctx->opt.precision != QMCB_MIXED_BF16STORE;
   auto next_proposal = [&](MoveArgs &a, long step, int s, int count) {
     if (!fuse_propose || s + 1 >= count) return;
     a.propose_next = 1;
