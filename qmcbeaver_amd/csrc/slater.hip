@@ -652,12 +652,11 @@ hipError_t launch_traces_all(const SlaterArgs &a, const int *walkers, int num, h
 // inverse element on the fly and accumulates w, T1, T2 per column (9 registers).
 // Thread mapping: c = t % NC (electron column), sl = t / NC (row slice).
 // Rows per software-pipeline stage of the read-only stream (2U rows in
-// flight per lane).  For the 128-column tile the deeper pipelines raise the
-// register count above 128 (FP32 derivatives: 131 VGPRs, 3 waves per SIMD;
-// FP64: 213, 2 waves per SIMD) with the streaming rate unchanged, which frees
-// wave slots for the basis, Jastrow and GEMM kernels of the other walker
-// groups running beside sm_move (+7 % / +8 % walker-moves/s with three groups
-// at 256 electrons, equal with one group).
+// flight per lane).  For the FP64 128-column tile the deeper pipeline raises
+// the register count to about 190 (2 waves per SIMD) with the streaming rate
+// unchanged, which leaves wave slots to the basis and GEMM kernels of the
+// other walker groups running beside sm_move (+8 % walker-moves/s with three
+// groups at 256 electrons, equal with one group).
 #ifndef QMCB_SM_UNROLL_F32
 #define QMCB_SM_UNROLL_F32 4
 #endif
@@ -682,9 +681,15 @@ __host__ __device__ constexpr int sm_unroll() {
 // waves per SIMD the register allocation must allow: four everywhere (at
 // most 128 VGPRs, no spills), two for the FP64 128-column tile, whose deeper
 // unroll holds about 200
+// (the FP32 128-column tile at four waves and 4 rows per stage measured
+// 8.42-8.44 M walker-moves/s with three groups at 256 electrons, against
+// 8.32-8.45 M for three or two waves with 4, 6 or 8 rows)
+#ifndef QMCB_SM_WAVES_F32
+#define QMCB_SM_WAVES_F32 4
+#endif
 template <typename TS, int NC>
 __host__ __device__ constexpr int sm_waves() {
-  return (NC == 128 && sizeof(TS) == 8) ? 2 : 4;
+  return NC == 128 ? (sizeof(TS) == 8 ? 2 : (sizeof(TS) == 4 ? QMCB_SM_WAVES_F32 : 4)) : 4;
 }
 #ifndef QMCB_SM_NT
 #define QMCB_SM_NT 1
