@@ -10,6 +10,7 @@ with delayed Sherman-Morrison updates and periodic FP64 re-inversion).
 * Each walker's trajectory depends only on (seed, walker, step): the same
   walkers run alone or in a larger batch end bit-identical.
 """
+import os
 import sys
 
 import numpy as np
@@ -229,3 +230,57 @@ def test_vmc_next_proposal_in_sm_move_bit_identical(config, precision, W, groups
         np.testing.assert_array_equal(a[0], b[0])
         np.testing.assert_array_equal(a[1], b[1])
         assert a[2] == b[2] and a[3] == b[3]
+
+
+def unequal_spins(n_up=68, n_dn=12, seed=4):
+    """The 300-function organic system with 68 + 12 electrons: moving an
+    up-spin electron (96-column tile, 192 threads) forms the Jastrow terms
+    inside sm_move, moving a down-spin one (16-column tile, 64 threads < 80
+    electrons) takes the separate jastrow_move launch."""
+    p = dict(synth.make_system(1).pack())
+    rng = np.random.default_rng(seed)
+    q, _ = np.linalg.qr(rng.normal(size=(p["n_basis"], n_up + n_dn)))
+    p["n_up"], p["n_dn"] = n_up, n_dn
+    p["mo_up"] = np.ascontiguousarray(q[:, :n_up].T)
+    p["mo_dn"] = np.ascontiguousarray(q[:, n_up:].T)
+    return p
+
+
+def test_unequal_spins_evaluate_and_walk():
+    """68 + 12 electrons: the FP64 all-electron evaluate against the oracle,
+    and a mixed-precision VMC (re-inversions, both Jastrow paths, the next
+    proposal drawn in the move kernel) whose resident state matches a fresh
+    evaluate and whose walk equals the one with a propose launch per step."""
+    from oracle.pyoracle import Oracle
+    p = unequal_spins()
+    N, W = 80, 256
+    rng = np.random.default_rng(8)
+    R0 = p["nuc_xyz"][rng.integers(0, p["n_nuc"], size=(W, N))] + 0.8 * rng.normal(size=(W, N, 3))
+    say("case unequal spins 68+12")
+    with Context(p, precision="fp64", max_walkers=W) as ctx:
+        a = ctx.evaluate(np.arange(W), R0)
+    o = Oracle(p).evaluate_all(R0[:8])
+    for k in ("log_psi", "e_local", "grad_lnpsi", "lap_ratio"):
+        err = np.max(np.abs(a.a[k][:8] - o.a[k]) / np.maximum(1.0, np.abs(o.a[k])))
+        assert err < 1e-10, (k, err)
+    out = []
+    for separate in ("0", "1"):
+        os.environ["QMCB_SEPARATE_PROPOSE"] = separate
+        try:
+            with Context(p, precision="fp32", max_walkers=W, seed=3, refresh_interval=16) as ctx:
+                ctx.evaluate(np.arange(W), R0)
+                st = ctx.vmc_steps(W, 0, 3 * N + 7, 0.02)
+                R = ctx.positions(W)
+                res = ctx.evaluate(np.arange(W), R, whichE=N - 1)  # zero-length proposal
+                out.append((R, ctx.sample_series(), st.sum_accept, res.a["e_local"].copy(),
+                            res.a["log_psi"].copy()))
+        finally:
+            del os.environ["QMCB_SEPARATE_PROPOSE"]
+    for x, y in zip(out[0][:3], out[1][:3]):
+        np.testing.assert_array_equal(x, y)
+    assert 0.05 < out[0][2] / (W * (3 * N + 7)) < 0.99
+    with Context(p, precision="fp32", max_walkers=W) as fresh:
+        ref = fresh.evaluate(np.arange(W), out[0][0])
+    for k, i in (("e_local", 3), ("log_psi", 4)):
+        err = np.max(np.abs(out[0][i] - ref.a[k]) / np.maximum(1.0, np.abs(ref.a[k])))
+        assert err < 1e-3, (k, err)
