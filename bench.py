@@ -536,7 +536,7 @@ def main():
     if bf16 is not None:
         out["bf16"] = {
             "description": "FP64 basis rounded to BF16, BF16 coefficients, "
-                           "v_mfma_f32_16x16x16_bf16 with FP32 accumulate, FP32 derivative "
+                           "v_mfma_f32_16x16x32_bf16 with FP32 accumulate, FP32 derivative "
                            "storage, FP64 inverse/log-det/E_L",
             "value": bf16["walker_moves_per_s"], "ms_per_step": bf16["ms_per_step"],
             "over_fp64": None if fp64 is None else
