@@ -5,9 +5,10 @@ Workload (default): the 256-electron synthetic molecule (128 up + 128 down,
 HBM, single-electron importance-sampled VMC moves.  One step = every walker
 proposes a move of electron (step mod 256), the full evaluate contract runs at
 the proposal (ln|Psi|, sign, grad ln Psi of all electrons, Laplacian, E_L) and
-the move is accepted or rejected on the device.  The mixed path (basis + MFMA
-orbital GEMM in FP32, Sherman-Morrison inverse/log-det/E_L in FP64, FP64
-re-inversion every 64 moves), with the walkers in 3 overlapping groups, is
+the move is accepted or rejected on the device.  The mixed path (basis in
+FP64 rounded to FP32 for the MFMA orbital GEMM, FP32 derivative matrices,
+Sherman-Morrison inverse/log-det/E_L in FP64, FP64 re-inversion every 256
+moves of a spin), with the walkers in 3 overlapping groups, is
 `value`; the plain FP64 path on the same GPU, the fp32store and bf16 modes,
 their per-quantity errors against the FP64 path, a serving run of 200
 Poisson-arriving jobs and the reference's own CPU evaluate on the host cores
@@ -352,7 +353,9 @@ def main():
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--tau", type=float, default=0.01)
     ap.add_argument("--block", type=int, default=16, help="steps per timed block")
-    ap.add_argument("--refresh", type=int, default=64)
+    ap.add_argument("--refresh", type=int, default=256,
+                    help="FP64 re-inversion of each walker's matrices of a spin every this many "
+                         "moves of that spin (mixed modes; the FP64 run never re-inverts)")
     ap.add_argument("--groups", type=int, default=3,
                     help="walker groups on overlapping streams (results are bit-identical); "
                          "-1: chosen by the library's timing")

@@ -284,3 +284,35 @@ def test_unequal_spins_evaluate_and_walk():
     for k, i in (("e_local", 3), ("log_psi", 4)):
         err = np.max(np.abs(out[0][i] - ref.a[k]) / np.maximum(1.0, np.abs(ref.a[k])))
         assert err < 1e-3, (k, err)
+
+
+@pytest.mark.parametrize("refresh", [0, 256])
+def test_mixed_resident_state_drift_256_electrons(refresh):
+    """The 256-electron molecule, FP32 derivative storage: after 12 sweeps of
+    single-electron moves with no FP64 re-inversion (the reference's own
+    single-electron mode) or one every 256 moves of a spin, the resident
+    state (Sherman-Morrison-updated FP64 inverse and delayed terms) gives
+    the local energy, ln|Psi| and grad ln Psi of a fresh evaluate of the same
+    positions to far better than the 1e-6 Ha per electron target."""
+    s = synth.make_system(2)
+    W, N = 512, s.n_elec
+    say("case resident drift, 256 electrons, refresh", refresh)
+    R0 = synth.initial_walkers(s, W, 5)
+    ids = np.arange(W)
+    with Context(s.pack(), precision="fp32", max_walkers=W, refresh_interval=refresh,
+                 seed=9) as ctx, \
+            Context(s.pack(), precision="fp32", max_walkers=W) as fresh:
+        ctx.evaluate(ids, R0)
+        done = 0
+        for chunk in (256, 2816):
+            ctx.vmc_steps(W, done, chunk, 0.01)
+            done += chunk
+            R = ctx.positions(W)
+            ref = fresh.evaluate(ids, R)
+            assert not ref["singular"].any()
+            res = ctx.evaluate(ids, R, whichE=N - 1)  # zero-length move of the last electron
+            errs = {k: rel(res[k], ref[k]) for k in ("e_local", "log_psi", "grad_lnpsi")}
+            say("steps", done, errs)
+            assert errs["e_local"] < 1e-9 and errs["log_psi"] < 1e-10, errs
+            assert errs["grad_lnpsi"] < 1e-7, errs
+            ctx.accept(ids, np.zeros(W, dtype=np.uint8))
