@@ -117,7 +117,7 @@ class JobRunner:
     """Runs one job's blocks on a context of its own (the device path)."""
 
     def __init__(self, precision: str = "fp32", tau_vmc: float = 0.1, tau_dmc: float = 0.01,
-                 refresh: int = 64, basis_scratch_bytes: int = 256 << 20):
+                 refresh: int = 256, basis_scratch_bytes: int = 256 << 20):
         self.precision = precision
         self.tau_vmc, self.tau_dmc = tau_vmc, tau_dmc
         self.refresh = refresh if precision != "fp64" else 0
@@ -363,7 +363,7 @@ class BatchedDMC:
             ne, W, sd = self.comp[i]
             s = job_molecule(ne, sd)
             c = Context(s.pack(), precision=self.precision, max_walkers=2 * W,
-                        refresh_interval=64 if self.precision != "fp64" else 0, seed=sd,
+                        refresh_interval=256 if self.precision != "fp64" else 0, seed=sd,
                         basis_scratch_bytes=self.scratch, drift_mod=DRIFT,
                         replace_cusps=self.cusps)
             self.ctx[i] = c
