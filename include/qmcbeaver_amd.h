@@ -136,10 +136,12 @@ typedef struct qmcb_options {
                                bit-identical for every value.  0 or 1 = one
                                group; 3 is fastest at 8192 walkers of 256
                                electrons (12 % over one group).  -1:
-                               qmcb_vmc_steps times 16 steps of each of 1, 2
-                               and 3 groups (windows without a re-inversion,
-                               over as many calls as needed, one group until
-                               then) and keeps the fastest, again when the
+                               qmcb_vmc_steps times windows of 16 steps (S
+                               when the refresh phase S above is 17-32) of
+                               each of 1, 2 and 3 groups (an untimed round,
+                               then two timed rounds, over as many calls as
+                               needed, one group until then) and keeps the
+                               fastest, again when the
                                batch changes by more than 2x; qmcb_dmc_steps
                                uses that choice                              */
   int32_t old_walker_acceptance; /* old_walker_acceptance_parameter of the

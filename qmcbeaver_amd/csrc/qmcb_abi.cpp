@@ -1571,10 +1571,13 @@ static qmcb_status series_reserve(qmcb_ctx *ctx, int64_t first_step, int32_t n_s
 // contiguous piece of rmap.  rmap is copied to d_rmap; the caller
 // synchronises the stream before rmap changes again.
 constexpr int kRefreshBatch = 1024;
+static int refresh_phase(int K, int num) {
+  return std::max(1, std::min(K, (int)(((long)K * kRefreshBatch + std::max(num, 1) - 1) /
+                                       std::max(num, 1))));
+}
 static qmcb_status refresh_plan(qmcb_ctx *ctx, const int *slots, int num) {
   const int K = ctx->opt.refresh_interval;
-  const int S = std::max(1, std::min(K, (int)(((long)K * kRefreshBatch + std::max(num, 1) - 1) /
-                                              std::max(num, 1))));
+  const int S = refresh_phase(K, num);
   const int P = (K + S - 1) / S;
   ctx->rmap.assign(std::max(num, 1), 0);
   ctx->rpos.assign(std::max(num, 1), 0);
@@ -1922,9 +1925,13 @@ qmcb_status qmcb_vmc_steps(qmcb_ctx *ctx, int32_t num, int64_t first_step, int32
     // (it absorbs the GEMM tuning of each candidate's group batch and the
     // first launches), then two timed rounds, interleaved, keeping each
     // candidate's faster window.  The count changes no result bit, so the
-    // windows are ordinary steps; every window carries its share of the
-    // staggered refresh.
-    constexpr int kTune = 16;
+    // windows are ordinary steps.  A window is 16 steps, or S steps when the
+    // refresh phase S (one staggered re-inversion launch every S moves of a
+    // spin) is 17-32: windows whose length is a multiple of S hold equally
+    // many launches, others may differ by one.
+    const int S_ref = ctx->opt.refresh_interval > 0
+                          ? refresh_phase(ctx->opt.refresh_interval, num) : 1;
+    const int kTune = S_ref > 16 && S_ref <= 32 ? S_ref : 16;
     auto valid = [&]() {
       return ctx->auto_groups > 0 && num <= 2 * ctx->auto_groups_num &&
              2 * num >= ctx->auto_groups_num;

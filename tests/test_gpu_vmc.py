@@ -147,19 +147,21 @@ def test_vmc_walker_groups_bit_identical(precision):
         np.testing.assert_array_equal(o[3], out[0][3])
 
 
-def test_vmc_walker_groups_chosen_by_timing():
+@pytest.mark.parametrize("W,refresh,steps", [(1024, 64, 200), (8192, 256, 320)])
+def test_vmc_walker_groups_chosen_by_timing(W, refresh, steps):
     """walker_groups = -1: the first long call times windows of 1, 2 and 3
     groups inside its own steps (an untimed round, then two timed rounds)
     and keeps the fastest; the trajectory, the call's sums and the
     per-sweep series equal the one-group run bit for bit, and the next call
-    keeps the choice."""
+    keeps the choice.  At 8192 walkers and a re-inversion every 256 moves the
+    windows are one refresh phase (32 steps) long."""
     s = synth.make_system(1)
-    W, N, steps = 1024, s.n_elec, 200
+    N = s.n_elec
     R0 = synth.initial_walkers(s, W, 29)
     out = []
     for groups in (1, -1):
         say("vmc walker groups", groups)
-        with Context(s.pack(), precision="fp32", max_walkers=W, seed=6, refresh_interval=64,
+        with Context(s.pack(), precision="fp32", max_walkers=W, seed=6, refresh_interval=refresh,
                      walker_groups=groups) as ctx:
             ctx.evaluate(np.arange(W), R0)
             assert ctx.walker_groups() == 0  # no steps yet
